@@ -1,0 +1,209 @@
+/*
+ * mdet TNF path: C ABI of the MI355X (gfx950) implementation of MdeT's
+ * tetranucleotide-frequency contig-graph path.
+ *
+ * The reference has no library boundary: its path is a set of functions and
+ * globals inside one CUDA program.  Each entry point below names the reference
+ * interface it replaces (file:line in jonathanavc/MdeT).
+ *
+ * Conventions
+ *   - Every function returns 0 (MDET_OK) or a negative MDET_E_* code; the
+ *     message of the last failure on the calling thread is mdet_last_error().
+ *     (The reference prints and exit(1)s, cuda/Cuda2.cu:341-354; the C++
+ *     drop-in in mdet/tnf.hpp restores that behaviour.)
+ *   - Arrays named *_dev are device (HBM) pointers; all others are host.
+ *   - A context owns one HIP stream and the scratch memory of the calls made
+ *     on it; one host thread drives a context at a time.  Calls are ordered
+ *     on the context's stream; results are valid after mdet_ctx_sync().
+ *   - Plain C types only: no HIP or torch types cross this boundary.
+ */
+#ifndef MDET_TNF_H
+#define MDET_TNF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDET_OK 0
+#define MDET_E_INVALID (-1)  /* bad argument */
+#define MDET_E_HIP (-2)      /* HIP runtime error (message names the call) */
+#define MDET_E_NOMEM (-3)    /* device allocation failed */
+#define MDET_E_RANGE (-4)    /* input outside the supported domain */
+#define MDET_E_STATE (-5)    /* object not ready for this call */
+#define MDET_E_CAPACITY (-6) /* caller buffer too small (message says how big) */
+
+#define MDET_NTNF 136 /* canonical tetranucleotides per TNF row */
+
+/* ------------------------------------------------------------- general -- */
+const char *mdet_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *mdet_last_error(void);
+
+/* The 136 column 4-mers of a TNF row, 4 chars each, in the reference's order
+ * (openmp/TNF.cpp:14-24 TN[]); out needs 545 bytes. */
+int mdet_tnf_columns(char *out, uint64_t cap);
+/* For each 2-bit 4-mer code (A=0 C=1 G=2 T=3, first base in the low bits:
+ * code = b0 | b1<<2 | b2<<4 | b3<<6) the TNF column it counts into.
+ * Replaces the lookup table TNmap_d (cuda/Cuda2.cu:149-159), re-derived for
+ * this library's base code. */
+int mdet_column_of_code(int16_t *out256);
+
+/* ------------------------------------------------------------- context -- */
+typedef struct mdet_ctx mdet_ctx;
+/* One context per (device, host thread).  Replaces the implicit per-thread
+ * device binding of cuda/Cuda2.cu:890-891. */
+int mdet_ctx_create(int device, mdet_ctx **out);
+int mdet_ctx_destroy(mdet_ctx *ctx);
+int mdet_ctx_sync(mdet_ctx *ctx);
+/* The context's hipStream_t, as an opaque pointer. */
+void *mdet_ctx_stream(mdet_ctx *ctx);
+
+/* Device memory and copies on the context's stream, for callers without
+ * HIP headers (the C++ drop-in, Python tests).  Copies are synchronous. */
+int mdet_dev_alloc(mdet_ctx *ctx, uint64_t bytes, void **out_dev);
+int mdet_dev_free(mdet_ctx *ctx, void *dev);
+int mdet_copy_h2d(mdet_ctx *ctx, void *dst_dev, const void *src, uint64_t bytes);
+int mdet_copy_d2h(mdet_ctx *ctx, void *dst, const void *src_dev, uint64_t bytes);
+
+/* ---------------------------------------------------------- host ingest -- */
+/* FASTA with kseq semantics (extra/kseq.h:190-229, as used by
+ * openmp/TNF.cpp:102-143): multi-line records joined, empty lines skipped,
+ * a trailing '\r' stripped per line, name = header up to the first blank.
+ * Pass 1 counts records and sequence bytes; pass 2 fills
+ * name_off/name_len[n_records] (offsets into buf), the joined sequences into
+ * seq_out and seq_off[n_records+1]. */
+int mdet_fasta_scan(const char *buf, uint64_t size, uint64_t *n_records, uint64_t *total_seq);
+int mdet_fasta_read(const char *buf, uint64_t size, uint64_t n_records, uint64_t *name_off,
+                    uint64_t *name_len, char *seq_out, uint64_t *seq_off);
+
+/* 2-bit packing.  Bases of all contigs are concatenated (contig r occupies
+ * global bases [seq_off[r], seq_off[r+1])), 16 per uint32 word, base p at
+ * bits 2*(p%16) of word p/16.  Every byte that is not A/C/G/T/a/c/g/t is an
+ * N: stored as 0 and listed in runs[2*k], runs[2*k+1] = [start, end) global
+ * base ranges, sorted and disjoint.  On MDET_E_CAPACITY *n_runs holds the
+ * capacity needed.  The packed stream replaces the char buffer the reference
+ * scans (cuda/Cuda2.cu:1826-1844; get_TNF's seqs_d, cuda/Cuda.cu:222). */
+int mdet_pack_ascii(const char *seq, const uint64_t *seq_off, uint64_t n, uint32_t *packed,
+                    uint64_t packed_words, uint64_t *runs, uint64_t runs_cap, uint64_t *n_runs);
+
+/* log10(min(length, 500000)) in FP64, the per-contig size feature of the
+ * pair model (cuda/Cuda2.cu:1535 logSizes). */
+int mdet_logsize_host(const uint64_t *lengths, uint64_t n, double *out);
+
+/* ------------------------------------------------------------ generator -- */
+/* Seeded order-3 Markov metagenome (no reference counterpart: the reference
+ * ships no data).  Deterministic for a config on any machine/thread count. */
+typedef struct mdet_gen_config {
+    uint64_t seed;
+    uint32_t n_genomes;
+    uint32_t n_run_max;  /* N runs have length U{1..n_run_max} */
+    double len_median;   /* log-normal contig lengths ... */
+    double len_sigma;
+    double len_min;      /* ... clipped to [len_min, len_max] */
+    double len_max;
+    double n_frac;       /* expected fraction of N bases */
+    double gc_lo, gc_hi; /* per-genome GC weight ~ U[gc_lo, gc_hi] */
+} mdet_gen_config;
+
+int mdet_gen_lengths(const mdet_gen_config *cfg, uint64_t n, uint64_t *lengths, uint32_t *genome);
+/* Packs contigs 0..n-1 at base_off[0..n] (lengths as mdet_gen_lengths gives),
+ * N runs as in mdet_pack_ascii.  threads <= 0: hardware concurrency (max 16). */
+int mdet_gen_packed(const mdet_gen_config *cfg, uint64_t n, const uint64_t *base_off, uint32_t *packed,
+                    uint64_t packed_words, uint64_t *runs, uint64_t runs_cap, uint64_t *n_runs,
+                    int threads);
+/* One contig as ASCII (A/C/G/T/N), identical bases to mdet_gen_packed. */
+int mdet_gen_ascii(const mdet_gen_config *cfg, uint64_t contig, char *out, uint64_t len);
+
+/* ------------------------------------------------------------- counting -- */
+/* TNF of n contigs from the 2-bit packed stream (device arrays):
+ *   packed_dev[ceil(n_bases/16)], runs_dev[2*n_runs] (may be NULL if
+ *   n_runs == 0), base_off_dev[n+1] (uint64 global base offsets).
+ * For each contig, every window of 4 consecutive bases of the contig that
+ * contains no N counts +1 in its canonical column; counts_dev[n*136]
+ * (optional, may be NULL) receives the integer counts and tnf_dev[n*136]
+ * the row c / sqrt(sum c^2) rounded once from FP64 to FP32 (0/0 = NaN for a
+ * contig with no valid window, as openmp/TNF.cpp:208-217).
+ * Replaces TNF_CAL (cuda/Cuda2.cu:1823-1864), get_TNF (cuda/Cuda.cu:222-269)
+ * and launch_tnf_kernel (cuda/Cuda.cu:1706-1736). */
+int mdet_tnf_count(mdet_ctx *ctx, const uint32_t *packed_dev, uint64_t n_bases, const uint64_t *runs_dev,
+                   uint64_t n_runs, const uint64_t *base_off_dev, uint32_t n, uint32_t *counts_dev,
+                   float *tnf_dev);
+
+/* Same result from raw bytes: contig r is seqs_dev[index_dev[r] ..
+ * index_dev[n+r]) (get_TNF's [begin | end] index, cuda/Cuda.cu:222-226),
+ * case-insensitive ACGT, anything else voids the windows that contain it. */
+int mdet_tnf_count_ascii(mdet_ctx *ctx, const char *seqs_dev, const uint64_t *index_dev, uint32_t n,
+                         uint32_t *counts_dev, float *tnf_dev);
+
+/* log10(min(len, 500000)) on the device, for lengths_dev[n] -> out_dev[n]. */
+int mdet_logsize(mdet_ctx *ctx, const uint64_t *lengths_dev, uint32_t n, double *out_dev);
+
+/* ---------------------------------------------------------------- pairs -- */
+/* Precision of the all-pairs distance stage; every mode yields the same edge
+ * set (see mdet_tnf_edges). */
+#define MDET_EXACT_F32 0   /* FP32 sum of squared differences for every pair */
+#define MDET_BF16_REFINE 1 /* BF16 MFMA Gram prefilter + FP32 refinement */
+#define MDET_F16_REFINE 2  /* FP16 MFMA Gram prefilter + FP32 refinement */
+
+typedef struct mdet_edges {
+    uint64_t count;
+    /* Device arrays owned by the context, valid until the next
+     * mdet_tnf_edges call on it or mdet_ctx_destroy.  Sorted by (from, to),
+     * from < to.  s = 1 - 1/(1+exp(preProb)) in FP32. */
+    const uint32_t *from_dev;
+    const uint32_t *to_dev;
+    const float *s_dev;
+} mdet_edges;
+
+typedef struct mdet_pair_stats {
+    uint64_t pairs_total;      /* n(n-1)/2 */
+    uint64_t pairs_evaluated;  /* pairs whose distance was computed (all modes: pairs_total) */
+    uint64_t survivors;        /* pairs passed from the prefilter to FP32 evaluation */
+    uint64_t candidates;       /* pairs with FP32 preProb above threshold - band */
+    uint64_t adjudicated;      /* pairs decided by the FP64 tie-breaker */
+    uint64_t rows_capped;      /* rows with more than max_edges candidates */
+    double ms_prefilter;       /* device time of each stage (HIP events) */
+    double ms_refine;
+    double ms_select;
+    double ms_total;
+} mdet_pair_stats;
+
+/*
+ * The thresholded, per-row-capped TNF edge list of n contigs:
+ *   tnf_dev[n*136] FP32 rows (unit L2 norm), logsize_dev[n] FP64 log10 sizes
+ *   with every size >= 1500 (the reference refuses minContig < 1500,
+ *   cuda/Cuda2.cu:1329-1332; the model is only monotone in d there).
+ * Edge (i, j), i < j, is emitted iff preProb(i,j) > log(1/(1-cutoff) - 1)
+ * and j is among the max_edges highest-similarity partners of row i
+ * (max_edges == 0: no cap), with preProb as cal_tnf_pre_dist_d computes it.
+ * Replaces gen_tnf_graph (cuda/Cuda2.cu:861-970) with get_tnf_graph2
+ * (cuda/Cuda2.cu:243-262) and the host heaps of lines 896-946; stats may be
+ * NULL.
+ */
+int mdet_tnf_edges(mdet_ctx *ctx, const float *tnf_dev, const double *logsize_dev, uint32_t n, double cutoff,
+                   uint32_t max_edges, int precision, mdet_edges *out, mdet_pair_stats *stats);
+
+/* The FP32 form of the model the device evaluates: tensor Chebyshev
+ * coefficients (4 x 8 x 7 floats: b1, c1, b2, c2; row p = degree in the
+ * min-size variable) of the logistic polynomials of cal_tnf_dist
+ * (cuda/Cuda2.cu:170-188, 747-771) on [log10 1500, log10 500000]^2 (domain2),
+ * and sum |C| of each (abs_sum4).  Host-only; abs_sum4/domain2 may be NULL. */
+int mdet_model_cheb(float *out, double *abs_sum4, double *domain2);
+
+/*
+ * preProb of listed pairs (device arrays pi_dev[np], pj_dev[np]) in the
+ * library's FP32 evaluator (pre32_dev, may be NULL) and its FP64 tie-breaker
+ * (pre64_dev, may be NULL).  The device function cal_tnf_pre_dist_d
+ * (cuda/Cuda2.cu:192-241) exposed for checking.
+ */
+int mdet_tnf_preprob(mdet_ctx *ctx, const float *tnf_dev, const double *logsize_dev, uint32_t n,
+                     const uint32_t *pi_dev, const uint32_t *pj_dev, uint64_t np, float *pre32_dev,
+                     double *pre64_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDET_TNF_H */

@@ -1,0 +1,320 @@
+"""Python binding of the mdet TNF library (libmdet_tnf.so) over its C ABI.
+
+This is a thin ctypes layer used by the tests and bench.py; the product is the
+C ABI (include/mdet/tnf.h) and the C++ drop-in (include/mdet/tnf.hpp).  The
+library is loaded from this directory only; if it is missing the import of any
+device entry point raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdet_tnf.so")
+NTNF = 136
+
+EXACT_F32, BF16_REFINE, F16_REFINE = 0, 1, 2
+PRECISIONS = {"exact_f32": EXACT_F32, "bf16_refine": BF16_REFINE, "f16_refine": F16_REFINE}
+
+
+class MdetError(RuntimeError):
+    pass
+
+
+class GenConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("n_genomes", C.c_uint32), ("n_run_max", C.c_uint32),
+                ("len_median", C.c_double), ("len_sigma", C.c_double), ("len_min", C.c_double),
+                ("len_max", C.c_double), ("n_frac", C.c_double), ("gc_lo", C.c_double), ("gc_hi", C.c_double)]
+
+
+class Edges(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("from_dev", C.c_void_p), ("to_dev", C.c_void_p), ("s_dev", C.c_void_p)]
+
+
+class PairStats(C.Structure):
+    _fields_ = [("pairs_total", C.c_uint64), ("pairs_evaluated", C.c_uint64), ("survivors", C.c_uint64),
+                ("candidates", C.c_uint64), ("adjudicated", C.c_uint64), ("rows_capped", C.c_uint64),
+                ("ms_prefilter", C.c_double), ("ms_refine", C.c_double), ("ms_select", C.c_double),
+                ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+# name -> (restype, argtypes); every function returns an int status unless noted
+_SIGS = {
+    "mdet_version": (C.c_char_p, []),
+    "mdet_last_error": (C.c_char_p, []),
+    "mdet_tnf_columns": (C.c_int, [C.c_char_p, C.c_uint64]),
+    "mdet_column_of_code": (C.c_int, [C.c_void_p]),
+    "mdet_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "mdet_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "mdet_ctx_sync": (C.c_int, [C.c_void_p]),
+    "mdet_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "mdet_dev_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "mdet_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mdet_copy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mdet_copy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mdet_fasta_scan": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "mdet_fasta_read": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "mdet_pack_ascii": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                  C.c_uint64, C.POINTER(C.c_uint64)]),
+    "mdet_logsize_host": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "mdet_gen_lengths": (C.c_int, [C.POINTER(GenConfig), C.c_uint64, C.c_void_p, C.c_void_p]),
+    "mdet_gen_packed": (C.c_int, [C.POINTER(GenConfig), C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
+                                  C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_int]),
+    "mdet_gen_ascii": (C.c_int, [C.POINTER(GenConfig), C.c_uint64, C.c_void_p, C.c_uint64]),
+    "mdet_model_cheb": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mdet_tnf_count": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                 C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mdet_tnf_count_ascii": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mdet_logsize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mdet_tnf_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double, C.c_uint32,
+                                 C.c_int, C.POINTER(Edges), C.POINTER(PairStats)]),
+    "mdet_tnf_preprob": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.c_uint64, C.c_void_p, C.c_void_p]),
+}
+
+
+def lib():
+    """Load libmdet_tnf.so from this package directory (raises if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MdetError(f"{LIB_PATH} is not built; run `python -m mdet_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        msg = lib().mdet_last_error().decode(errors="replace")
+        raise MdetError(f"{what} failed with code {rc}: {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ----------------------------------------------------------------- host side
+def gen_config(seed=1, n_genomes=20, len_median=6000.0, len_sigma=1.0, len_min=2500.0, len_max=100000.0,
+               n_frac=0.005, n_run_max=50, gc_lo=0.3, gc_hi=0.7) -> GenConfig:
+    return GenConfig(seed, n_genomes, n_run_max, len_median, len_sigma, len_min, len_max, n_frac, gc_lo, gc_hi)
+
+
+def gen_lengths(cfg: GenConfig, n: int):
+    lens = np.zeros(n, np.uint64)
+    gid = np.zeros(n, np.uint32)
+    _check(lib().mdet_gen_lengths(C.byref(cfg), n, _ptr(lens), _ptr(gid)), "mdet_gen_lengths")
+    return lens, gid
+
+
+def gen_packed(cfg: GenConfig, n: int, threads: int = 0):
+    """(packed uint32, runs uint64[2k], base_off uint64[n+1], lengths, genome ids)."""
+    lens, gid = gen_lengths(cfg, n)
+    off = np.zeros(n + 1, np.uint64)
+    np.cumsum(lens, out=off[1:])
+    total = int(off[-1])
+    words = (total + 15) // 16
+    packed = np.zeros(max(words, 1), np.uint32)
+    cap = max(1024, total // 500)
+    nr = C.c_uint64(0)
+    runs = np.zeros(2 * cap, np.uint64)
+    rc = lib().mdet_gen_packed(C.byref(cfg), n, _ptr(off), _ptr(packed), packed.size, _ptr(runs), cap,
+                               C.byref(nr), threads)
+    if rc == -6:
+        cap = int(nr.value) + 16
+        runs = np.zeros(2 * cap, np.uint64)
+        rc = lib().mdet_gen_packed(C.byref(cfg), n, _ptr(off), _ptr(packed), packed.size, _ptr(runs), cap,
+                                   C.byref(nr), threads)
+    _check(rc, "mdet_gen_packed")
+    return packed, runs[: 2 * nr.value].copy(), off, lens, gid
+
+
+def gen_ascii(cfg: GenConfig, contig: int, length: int) -> bytes:
+    buf = C.create_string_buffer(int(length))
+    _check(lib().mdet_gen_ascii(C.byref(cfg), contig, buf, length), "mdet_gen_ascii")
+    return buf.raw[:length]
+
+
+def fasta_read(data: bytes):
+    """[(name, seq bytes)] with kseq semantics."""
+    nr, tot = C.c_uint64(0), C.c_uint64(0)
+    _check(lib().mdet_fasta_scan(data, len(data), C.byref(nr), C.byref(tot)), "mdet_fasta_scan")
+    n = nr.value
+    no = np.zeros(max(n, 1), np.uint64)
+    nl = np.zeros(max(n, 1), np.uint64)
+    so = np.zeros(n + 1, np.uint64)
+    seq = C.create_string_buffer(max(int(tot.value), 1))
+    _check(lib().mdet_fasta_read(data, len(data), n, _ptr(no), _ptr(nl), seq, _ptr(so)), "mdet_fasta_read")
+    raw = seq.raw
+    return [(data[int(no[i]): int(no[i] + nl[i])], raw[int(so[i]): int(so[i + 1])]) for i in range(n)]
+
+
+def pack_ascii(seqs):
+    """Pack a list of byte strings: (packed, runs, base_off)."""
+    off = np.zeros(len(seqs) + 1, np.uint64)
+    np.cumsum([len(s) for s in seqs], out=off[1:])
+    buf = np.frombuffer(b"".join(seqs) or b"\0", np.uint8)
+    total = int(off[-1])
+    packed = np.zeros(max((total + 15) // 16, 1), np.uint32)
+    cap = max(16, total // 2 + 1)
+    runs = np.zeros(2 * cap, np.uint64)
+    nr = C.c_uint64(0)
+    _check(lib().mdet_pack_ascii(_ptr(buf), _ptr(off), len(seqs), _ptr(packed), packed.size, _ptr(runs), cap,
+                                 C.byref(nr)), "mdet_pack_ascii")
+    return packed, runs[: 2 * nr.value].copy(), off
+
+
+def logsize_host(lengths) -> np.ndarray:
+    lens = np.ascontiguousarray(lengths, np.uint64)
+    out = np.zeros(len(lens), np.float64)
+    _check(lib().mdet_logsize_host(_ptr(lens), len(lens), _ptr(out)), "mdet_logsize_host")
+    return out
+
+
+def tnf_columns() -> list[str]:
+    buf = C.create_string_buffer(545)
+    _check(lib().mdet_tnf_columns(buf, 545), "mdet_tnf_columns")
+    s = buf.value.decode()
+    return [s[4 * i: 4 * i + 4] for i in range(NTNF)]
+
+
+def column_of_code() -> np.ndarray:
+    out = np.zeros(256, np.int16)
+    _check(lib().mdet_column_of_code(_ptr(out)), "mdet_column_of_code")
+    return out
+
+
+def model_cheb():
+    out = np.zeros((4, 8, 7), np.float32)
+    s = np.zeros(4, np.float64)
+    d = np.zeros(2, np.float64)
+    _check(lib().mdet_model_cheb(_ptr(out), _ptr(s), _ptr(d)), "mdet_model_cheb")
+    return out, s, d
+
+
+# --------------------------------------------------------------- device side
+class DevArray:
+    """A device buffer owned by a Ctx (freed with it or by .free())."""
+
+    def __init__(self, ctx: "Ctx", nbytes: int, dtype, shape):
+        self.ctx, self.nbytes, self.dtype, self.shape = ctx, int(nbytes), np.dtype(dtype), tuple(shape)
+        p = C.c_void_p()
+        _check(lib().mdet_dev_alloc(ctx.h, max(self.nbytes, 1), C.byref(p)), "mdet_dev_alloc")
+        self.ptr = p.value
+        ctx._owned.append(self)
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.sync()
+        if self.nbytes:
+            _check(lib().mdet_copy_d2h(self.ctx.h, _ptr(out), C.c_void_p(self.ptr), self.nbytes), "mdet_copy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mdet_dev_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class Ctx:
+    """One mdet context (device + stream + scratch)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().mdet_ctx_create(device, C.byref(h)), "mdet_ctx_create")
+        self.h = h
+        self._owned: list[DevArray] = []
+
+    def close(self):
+        if self.h:
+            for a in self._owned:
+                a.free()
+            lib().mdet_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        _check(lib().mdet_ctx_sync(self.h), "mdet_ctx_sync")
+
+    def empty(self, shape, dtype) -> DevArray:
+        shape = (shape,) if np.isscalar(shape) else tuple(shape)
+        return DevArray(self, int(np.prod(shape)) * np.dtype(dtype).itemsize, dtype, shape)
+
+    def to_device(self, a: np.ndarray) -> DevArray:
+        a = np.ascontiguousarray(a)
+        d = DevArray(self, a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            _check(lib().mdet_copy_h2d(self.h, C.c_void_p(d.ptr), _ptr(a), a.nbytes), "mdet_copy_h2d")
+        return d
+
+    # -- TNF path
+    def tnf_count(self, packed: DevArray, n_bases: int, runs: DevArray | None, n_runs: int, base_off: DevArray,
+                  n: int, counts: DevArray | None, tnf: DevArray):
+        _check(lib().mdet_tnf_count(self.h, C.c_void_p(packed.ptr), n_bases,
+                                    C.c_void_p(runs.ptr if runs is not None else None), n_runs,
+                                    C.c_void_p(base_off.ptr), n,
+                                    C.c_void_p(counts.ptr if counts is not None else None), C.c_void_p(tnf.ptr)),
+               "mdet_tnf_count")
+
+    def tnf_count_ascii(self, seqs: DevArray, index: DevArray, n: int, counts: DevArray | None, tnf: DevArray):
+        _check(lib().mdet_tnf_count_ascii(self.h, C.c_void_p(seqs.ptr), C.c_void_p(index.ptr), n,
+                                          C.c_void_p(counts.ptr if counts is not None else None),
+                                          C.c_void_p(tnf.ptr)), "mdet_tnf_count_ascii")
+
+    def logsize(self, lengths: DevArray, n: int, out: DevArray):
+        _check(lib().mdet_logsize(self.h, C.c_void_p(lengths.ptr), n, C.c_void_p(out.ptr)), "mdet_logsize")
+
+    def tnf_edges(self, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int,
+                  precision: int = EXACT_F32, fetch: bool = True):
+        e = Edges()
+        st = PairStats()
+        _check(lib().mdet_tnf_edges(self.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n, cutoff, max_edges,
+                                    precision, C.byref(e), C.byref(st)), "mdet_tnf_edges")
+        if not fetch:
+            return e.count, st
+        self.sync()
+        k = int(e.count)
+        f = np.zeros(k, np.uint32)
+        t = np.zeros(k, np.uint32)
+        s = np.zeros(k, np.float32)
+        if k:
+            _check(lib().mdet_copy_d2h(self.h, _ptr(f), C.c_void_p(e.from_dev), 4 * k), "copy")
+            _check(lib().mdet_copy_d2h(self.h, _ptr(t), C.c_void_p(e.to_dev), 4 * k), "copy")
+            _check(lib().mdet_copy_d2h(self.h, _ptr(s), C.c_void_p(e.s_dev), 4 * k), "copy")
+        return f, t, s, st
+
+    def tnf_preprob(self, tnf: DevArray, logsize: DevArray, n: int, pi: np.ndarray, pj: np.ndarray):
+        pi_d = self.to_device(np.ascontiguousarray(pi, np.uint32))
+        pj_d = self.to_device(np.ascontiguousarray(pj, np.uint32))
+        m = len(pi)
+        p32 = self.empty(m, np.float32)
+        p64 = self.empty(m, np.float64)
+        _check(lib().mdet_tnf_preprob(self.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n,
+                                      C.c_void_p(pi_d.ptr), C.c_void_p(pj_d.ptr), m, C.c_void_p(p32.ptr),
+                                      C.c_void_p(p64.ptr)), "mdet_tnf_preprob")
+        out = p32.numpy(), p64.numpy()
+        for a in (pi_d, pj_d, p32, p64):
+            a.free()
+        return out

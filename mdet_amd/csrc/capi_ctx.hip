@@ -1,0 +1,142 @@
+// Context lifetime, scratch memory and plain copies of the C ABI.
+#include "hip_common.h"
+
+namespace mdet {
+
+int check_ctx(mdet_ctx* ctx) {
+    if (!ctx) MDET_FAIL(MDET_E_INVALID, "null mdet_ctx");
+    MDET_HIP(hipSetDevice(ctx->device));
+    return MDET_OK;
+}
+
+int ctx_buf(mdet_ctx* ctx, const char* key, size_t bytes, void** out) {
+    mdet_ctx::Buf& b = ctx->bufs[key];
+    if (bytes == 0) bytes = 256;
+    if (b.bytes < bytes) {
+        if (b.p) {
+            MDET_HIP(hipStreamSynchronize(ctx->stream));
+            MDET_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        size_t want = bytes + bytes / 8;  // headroom against repeated growth
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            MDET_FAIL(MDET_E_NOMEM, "device allocation of %zu bytes for '%s' failed: %s", want, key,
+                      hipGetErrorString(e));
+        }
+        b.bytes = want;
+    }
+    *out = b.p;
+    return MDET_OK;
+}
+
+int ctx_host_stage(mdet_ctx* ctx, size_t bytes, void** out) {
+    if (ctx->host_stage_bytes < bytes) {
+        if (ctx->host_stage) MDET_HIP(hipHostFree(ctx->host_stage));
+        ctx->host_stage = nullptr;
+        MDET_HIP(hipHostMalloc(&ctx->host_stage, bytes, hipHostMallocDefault));
+        ctx->host_stage_bytes = bytes;
+    }
+    *out = ctx->host_stage;
+    return MDET_OK;
+}
+
+int ctx_event(mdet_ctx* ctx, int idx, hipEvent_t* out) {
+    while ((int)ctx->events.size() <= idx) {
+        hipEvent_t e;
+        MDET_HIP(hipEventCreate(&e));
+        ctx->events.push_back(e);
+    }
+    *out = ctx->events[idx];
+    return MDET_OK;
+}
+
+}  // namespace mdet
+
+using namespace mdet;
+
+extern "C" {
+
+int mdet_ctx_create(int device, mdet_ctx** out) {
+    if (!out) MDET_FAIL(MDET_E_INVALID, "mdet_ctx_create: null output");
+    *out = nullptr;
+    int ndev = 0;
+    MDET_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) MDET_FAIL(MDET_E_INVALID, "mdet_ctx_create: device %d of %d", device, ndev);
+    MDET_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    MDET_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+        MDET_FAIL(MDET_E_STATE, "mdet_ctx_create: device %d is %s; this build targets gfx950", device,
+                  prop.gcnArchName);
+    mdet_ctx* c = new mdet_ctx();
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        MDET_FAIL(MDET_E_HIP, "hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return MDET_OK;
+}
+
+int mdet_ctx_destroy(mdet_ctx* ctx) {
+    if (!ctx) return MDET_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MDET_OK;
+}
+
+int mdet_ctx_sync(mdet_ctx* ctx) {
+    MDET_TRY(check_ctx(ctx));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    return MDET_OK;
+}
+
+void* mdet_ctx_stream(mdet_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mdet_dev_alloc(mdet_ctx* ctx, uint64_t bytes, void** out_dev) {
+    MDET_TRY(check_ctx(ctx));
+    if (!out_dev) MDET_FAIL(MDET_E_INVALID, "mdet_dev_alloc: null output");
+    hipError_t e = hipMalloc(out_dev, bytes ? bytes : 1);
+    if (e != hipSuccess) MDET_FAIL(MDET_E_NOMEM, "hipMalloc(%llu) failed: %s", (unsigned long long)bytes, hipGetErrorString(e));
+    return MDET_OK;
+}
+
+int mdet_dev_free(mdet_ctx* ctx, void* dev) {
+    MDET_TRY(check_ctx(ctx));
+    if (dev) {
+        MDET_HIP(hipStreamSynchronize(ctx->stream));
+        MDET_HIP(hipFree(dev));
+    }
+    return MDET_OK;
+}
+
+int mdet_copy_h2d(mdet_ctx* ctx, void* dst_dev, const void* src, uint64_t bytes) {
+    MDET_TRY(check_ctx(ctx));
+    if (!bytes) return MDET_OK;
+    if (!dst_dev || !src) MDET_FAIL(MDET_E_INVALID, "mdet_copy_h2d: null pointer");
+    MDET_HIP(hipMemcpyAsync(dst_dev, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    return MDET_OK;
+}
+
+int mdet_copy_d2h(mdet_ctx* ctx, void* dst, const void* src_dev, uint64_t bytes) {
+    MDET_TRY(check_ctx(ctx));
+    if (!bytes) return MDET_OK;
+    if (!dst || !src_dev) MDET_FAIL(MDET_E_INVALID, "mdet_copy_d2h: null pointer");
+    MDET_HIP(hipMemcpyAsync(dst, src_dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    return MDET_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Device-side plumbing shared by the HIP translation units: error checks,
+// the context object, grow-only scratch buffers and stage timers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "host_common.h"
+
+#define MDET_HIP(call)                                                                        \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            ::mdet::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, \
+                              __LINE__);                                                      \
+            return MDET_E_HIP;                                                                \
+        }                                                                                     \
+    } while (0)
+
+#define MDET_TRY(expr)               \
+    do {                             \
+        int _r = (expr);             \
+        if (_r != MDET_OK) return _r; \
+    } while (0)
+
+struct mdet_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::unordered_map<std::string, Buf> bufs;  // scratch, grown on demand
+    std::vector<hipEvent_t> events;
+    // pinned host staging for small device->host reads (counters)
+    void* host_stage = nullptr;
+    size_t host_stage_bytes = 0;
+    bool model_ready = false;  // FP32 model constants uploaded
+    int num_cus = 0;
+};
+
+namespace mdet {
+
+// Device scratch buffer `key` of at least `bytes` bytes, owned by the
+// context (contents are not preserved across growth).
+int ctx_buf(mdet_ctx* ctx, const char* key, size_t bytes, void** out);
+template <class T>
+int ctx_buf(mdet_ctx* ctx, const char* key, size_t count, T** out) {
+    void* p = nullptr;
+    int r = ctx_buf(ctx, key, count * sizeof(T), &p);
+    *out = static_cast<T*>(p);
+    return r;
+}
+int ctx_host_stage(mdet_ctx* ctx, size_t bytes, void** out);
+int ctx_event(mdet_ctx* ctx, int idx, hipEvent_t* out);
+int check_ctx(mdet_ctx* ctx);
+
+// Upload of the FP32 model constants (Chebyshev coefficients) to the device.
+int ensure_model(mdet_ctx* ctx);
+
+}  // namespace mdet

@@ -1,0 +1,302 @@
+// Host-side ingest of the TNF path: FASTA reading with kseq semantics, 2-bit
+// packing with an N-run list, log10 contig sizes, and the error channel.
+//
+// The reference reads FASTA either with kseq (openmp/TNF.cpp:102-143 through
+// extra/kseq.h:190-229) or with its own pread parser (cuda/Cuda2.cu:1436-1572).
+// This reader follows kseq, because openmp/TNF.cpp is the counting oracle.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "host_common.h"
+#include "tnf_tables.h"
+
+namespace mdet {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+void clear_error() { g_err.clear(); }
+
+static uint8_t code_of_char(char c) {
+    switch (c) {
+        case 'A': case 'a': return 0;
+        case 'C': case 'c': return 1;
+        case 'G': case 'g': return 2;
+        case 'T': case 't': return 3;
+        default: return 4;
+    }
+}
+
+struct AsciiTable {
+    uint8_t t[256];
+    AsciiTable() {
+        for (int c = 0; c < 256; ++c) t[c] = code_of_char((char)c);
+    }
+};
+static const AsciiTable g_ascii;
+
+static ColumnTables build_columns() {
+    ColumnTables t{};
+    for (int code = 0; code < 256; ++code) t.column_of_code[code] = kColumnMap.col[code];
+    t.ready = true;
+    return t;
+}
+
+const ColumnTables& column_tables() {
+    static const ColumnTables t = build_columns();
+    return t;
+}
+
+}  // namespace mdet
+
+using namespace mdet;
+
+extern "C" {
+
+const char* mdet_last_error(void) { return g_err.c_str(); }
+
+const char* mdet_version(void) { return "mdet-tnf 0.1 (gfx950)"; }
+
+int mdet_tnf_columns(char* out, uint64_t cap) {
+    if (!out || cap < 136 * 4 + 1) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_columns: need a buffer of 545 bytes");
+    memcpy(out, kTnColumnsCx, 136 * 4 + 1);
+    return MDET_OK;
+}
+
+int mdet_column_of_code(int16_t* out256) {
+    if (!out256) MDET_FAIL(MDET_E_INVALID, "mdet_column_of_code: null output");
+    const ColumnTables& t = column_tables();
+    for (int c = 0; c < 256; ++c) {
+        if (t.column_of_code[c] < 0) MDET_FAIL(MDET_E_STATE, "column table incomplete at code %d", c);
+        out256[c] = t.column_of_code[c];
+    }
+    return MDET_OK;
+}
+
+// ---------------------------------------------------------------- FASTA ----
+// kseq record loop (extra/kseq.h:190-229): skip to the first '>' or '@'; the
+// name is the header up to the first isspace(); the rest of the header line is
+// the comment; sequence lines are joined, empty lines skipped, and a trailing
+// '\r' is removed from the accumulated sequence after each line when the
+// accumulated length exceeds 1 (ks_getuntil2, extra/kseq.h:143).  A line that
+// starts with '>' '@' or '+' ends the record.  FASTQ ('+') is rejected.
+
+}  // extern "C"
+
+namespace {
+struct FastaCursor {
+    const char* b;
+    uint64_t n;
+    uint64_t i = 0;
+};
+
+// Returns false at end of input.  On true: name [name_off, name_off+name_len),
+// and the sequence bytes are delivered by append(line_ptr, len) calls.
+template <class Append>
+int next_record(FastaCursor& c, bool& have_header, uint64_t& name_off, uint64_t& name_len,
+                uint64_t& seq_len, Append&& append) {
+    const char* b = c.b;
+    if (!have_header) {
+        while (c.i < c.n && b[c.i] != '>' && b[c.i] != '@') ++c.i;
+        if (c.i >= c.n) return 0;
+        have_header = true;
+    }
+    if (b[c.i] == '@') return -1;
+    ++c.i;  // header char
+    name_off = c.i;
+    while (c.i < c.n && !isspace((unsigned char)b[c.i])) ++c.i;
+    name_len = c.i - name_off;
+    // kseq: when the name ends at EOF with no delimiter, ks_getuntil still
+    // returns the name.  Skip the rest of the header line.
+    if (c.i < c.n && b[c.i] != '\n') {
+        while (c.i < c.n && b[c.i] != '\n') ++c.i;
+    }
+    if (c.i < c.n) ++c.i;  // the '\n'
+    seq_len = 0;
+    while (c.i < c.n) {
+        char ch = b[c.i];
+        if (ch == '>' || ch == '@') {
+            have_header = true;
+            return 1;
+        }
+        if (ch == '+') return -1;
+        if (ch == '\n') {
+            ++c.i;
+            continue;
+        }
+        uint64_t s = c.i;
+        while (c.i < c.n && b[c.i] != '\n') ++c.i;
+        uint64_t len = c.i - s;  // >= 1: empty lines were skipped above
+        // kseq appends the line's first byte, then reads the rest with
+        // ks_getuntil2, which strips the '\r' -- unless that read found
+        // nothing before end of input (a 1-byte last line with no newline).
+        bool rest_read = (len > 1) || (c.i < c.n);
+        if (c.i < c.n) ++c.i;
+        uint64_t total = seq_len + len;
+        if (rest_read && total > 1 && b[s + len - 1] == '\r') {
+            append(b + s, len - 1);
+            seq_len = total - 1;
+        } else {
+            append(b + s, len);
+            seq_len = total;
+        }
+    }
+    have_header = false;
+    return 1;
+}
+}  // namespace
+
+extern "C" {
+
+int mdet_fasta_scan(const char* buf, uint64_t size, uint64_t* n_records, uint64_t* total_seq) {
+    if (!buf && size) MDET_FAIL(MDET_E_INVALID, "mdet_fasta_scan: null buffer");
+    if (!n_records || !total_seq) MDET_FAIL(MDET_E_INVALID, "mdet_fasta_scan: null output");
+    FastaCursor c{buf, size};
+    bool hh = false;
+    uint64_t nr = 0, tot = 0, no, nl, sl;
+    for (;;) {
+        int r = next_record(c, hh, no, nl, sl, [](const char*, uint64_t) {});
+        if (r < 0) MDET_FAIL(MDET_E_INVALID, "FASTQ input is not supported (record %llu)", (unsigned long long)nr);
+        if (r == 0) break;
+        ++nr;
+        tot += sl;
+    }
+    *n_records = nr;
+    *total_seq = tot;
+    return MDET_OK;
+}
+
+int mdet_fasta_read(const char* buf, uint64_t size, uint64_t n_records, uint64_t* name_off,
+                    uint64_t* name_len, char* seq_out, uint64_t* seq_off) {
+    if (!buf && size) MDET_FAIL(MDET_E_INVALID, "mdet_fasta_read: null buffer");
+    if (!name_off || !name_len || !seq_off || (!seq_out && size))
+        MDET_FAIL(MDET_E_INVALID, "mdet_fasta_read: null output");
+    FastaCursor c{buf, size};
+    bool hh = false;
+    uint64_t nr = 0, pos = 0, no, nl, sl;
+    seq_off[0] = 0;
+    for (;;) {
+        if (nr > n_records) MDET_FAIL(MDET_E_CAPACITY, "mdet_fasta_read: more records than n_records");
+        uint64_t start = pos;
+        int r = next_record(c, hh, no, nl, sl, [&](const char* p, uint64_t len) {
+            memcpy(seq_out + pos, p, len);
+            pos += len;
+        });
+        if (r < 0) MDET_FAIL(MDET_E_INVALID, "FASTQ input is not supported");
+        if (r == 0) break;
+        if (nr >= n_records) MDET_FAIL(MDET_E_CAPACITY, "mdet_fasta_read: more records than n_records");
+        pos = start + sl;  // a stripped '\r' was appended then dropped
+        name_off[nr] = no;
+        name_len[nr] = nl;
+        seq_off[nr + 1] = pos;
+        ++nr;
+    }
+    if (nr != n_records) MDET_FAIL(MDET_E_INVALID, "mdet_fasta_read: found %llu records, expected %llu",
+                                   (unsigned long long)nr, (unsigned long long)n_records);
+    return MDET_OK;
+}
+
+// ---------------------------------------------------------------- packing --
+// Packed layout: bases concatenated in contig order, no padding, 16 bases per
+// uint32 word, base p at bits 2*(p%16) of word p/16 (first base lowest).
+// Bases that are not A/C/G/T (any case) are stored as 0 and listed in the
+// N-run list as half-open [start, end) global base ranges, sorted.
+
+int mdet_pack_ascii(const char* seq, const uint64_t* seq_off, uint64_t n, uint32_t* packed,
+                    uint64_t packed_words, uint64_t* runs, uint64_t runs_cap, uint64_t* n_runs) {
+    if (!seq_off || !n_runs) MDET_FAIL(MDET_E_INVALID, "mdet_pack_ascii: null argument");
+    uint64_t total = seq_off[n];
+    if (total && (!seq || !packed)) MDET_FAIL(MDET_E_INVALID, "mdet_pack_ascii: null buffer");
+    if ((total + 15) / 16 > packed_words)
+        MDET_FAIL(MDET_E_CAPACITY, "mdet_pack_ascii: packed buffer holds %llu words, need %llu",
+                  (unsigned long long)packed_words, (unsigned long long)((total + 15) / 16));
+    const uint8_t* tab = g_ascii.t;
+    uint64_t nw = (total + 15) / 16;
+    // Words are independent: each thread packs a word range.
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nw < (1u << 16)) nt = 1;
+    std::vector<std::vector<uint64_t>> tr(nt);
+    auto work = [&](unsigned t) {
+        uint64_t w0 = nw * t / nt, w1 = nw * (t + 1) / nt;
+        std::vector<uint64_t>& r = tr[t];
+        bool in_run = false;
+        uint64_t run_s = 0;
+        for (uint64_t w = w0; w < w1; ++w) {
+            uint32_t v = 0;
+            uint64_t p0 = w * 16;
+            uint64_t pe = std::min(p0 + 16, total);
+            for (uint64_t p = p0; p < pe; ++p) {
+                uint8_t c = tab[(uint8_t)seq[p]];
+                if (c > 3) {
+                    if (!in_run) { in_run = true; run_s = p; }
+                    c = 0;
+                } else if (in_run) {
+                    r.push_back(run_s);
+                    r.push_back(p);
+                    in_run = false;
+                }
+                v |= (uint32_t)c << (2 * (p - p0));
+            }
+            packed[w] = v;
+        }
+        if (in_run) {
+            r.push_back(run_s);
+            r.push_back(std::min(w1 * 16, total));
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    // Merge thread lists, joining runs that touch at thread boundaries.
+    uint64_t cnt = 0;
+    uint64_t* out = runs;
+    for (unsigned t = 0; t < nt; ++t) {
+        for (size_t k = 0; k < tr[t].size(); k += 2) {
+            uint64_t s = tr[t][k], e = tr[t][k + 1];
+            if (cnt && out[2 * (cnt - 1) + 1] == s) {
+                out[2 * (cnt - 1) + 1] = e;
+                continue;
+            }
+            if (cnt >= runs_cap) {
+                // keep counting so the caller learns the size needed
+                ++cnt;
+                out = nullptr;
+                continue;
+            }
+            if (!out) { ++cnt; continue; }
+            out[2 * cnt] = s;
+            out[2 * cnt + 1] = e;
+            ++cnt;
+        }
+    }
+    *n_runs = cnt;
+    if (cnt > runs_cap) MDET_FAIL(MDET_E_CAPACITY, "mdet_pack_ascii: %llu N runs, capacity %llu",
+                                  (unsigned long long)cnt, (unsigned long long)runs_cap);
+    return MDET_OK;
+}
+
+// log10(min(len, 500000)) in FP64, cuda/Cuda2.cu:1535 (host libm, as the
+// reference computes it while parsing).
+int mdet_logsize_host(const uint64_t* lengths, uint64_t n, double* out) {
+    if ((!lengths || !out) && n) MDET_FAIL(MDET_E_INVALID, "mdet_logsize_host: null argument");
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t v = lengths[i] < 500000u ? lengths[i] : 500000u;
+        out[i] = log10((double)v);
+    }
+    return MDET_OK;
+}
+
+}  // extern "C"

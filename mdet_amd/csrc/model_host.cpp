@@ -1,0 +1,149 @@
+// Constants of the TNF pair model (the logistic models of cal_tnf_dist,
+// cuda/Cuda2.cu:170-241 and 721-779) in the two forms the device uses.
+//
+// 1. The monomial coefficient tables, for the FP64 tie-breaker, which repeats
+//    the reference's device arithmetic (cal_tnf_pre_dist_d).
+// 2. A tensor Chebyshev re-expansion of each of the four polynomials
+//    b1, c1, b2, c2 (x = min log size, y = max log size) on the square
+//    [log10 1500, log10 500000]^2, for the FP32 evaluator.  In the monomial
+//    basis the terms reach 1e8 while the values are 1..66, so FP32 evaluation
+//    loses everything; in the Chebyshev basis sum |C| is O(100) and FP32 keeps
+//    ~1e-5 absolute accuracy.  The expansion is computed once from the
+//    monomial tables by exact-degree interpolation at Gauss-Chebyshev nodes in
+//    x87 extended precision (degree <= 7 in x and <= 6 in y, so 8 x 7 nodes
+//    reproduce each polynomial exactly up to rounding).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "host_common.h"
+#include "model_tables.h"
+
+namespace mdet {
+
+// Monomial exponents (px, py) of each coefficient, in the order of the
+// reference's tables (cuda/Cuda2.cu:221-236 via the lw[] map of lines 201-219).
+static const int EXP_B1[18][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {0, 2}, {3, 0}, {0, 3}, {4, 0}, {0, 4},
+                                  {5, 0}, {0, 5}, {6, 0}, {7, 0}, {1, 1}, {2, 2}, {3, 3}, {4, 4}, {5, 5}};
+static const int EXP_B2[17][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {0, 2}, {3, 0}, {0, 3}, {4, 0}, {0, 4},
+                                  {5, 0}, {0, 5}, {6, 0}, {0, 6}, {2, 2}, {3, 3}, {4, 4}, {5, 5}};
+static const int EXP_C2[19][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {0, 2}, {3, 0}, {0, 3}, {4, 0}, {0, 4}, {5, 0},
+                                  {0, 5}, {6, 0}, {0, 6}, {7, 0}, {1, 1}, {2, 2}, {3, 3}, {4, 4}, {5, 5}};
+
+static long double eval_monomial(const double* coef, const int (*ex)[2], int nterm, long double x, long double y) {
+    long double s = 0;
+    for (int t = 0; t < nterm; ++t) {
+        long double m = coef[t];
+        for (int k = 0; k < ex[t][0]; ++k) m *= x;
+        for (int k = 0; k < ex[t][1]; ++k) m *= y;
+        s += m;
+    }
+    return s;
+}
+
+static void cheb_fit(const double* coef, const int (*ex)[2], int nterm, float out[MODEL_CX][MODEL_CY],
+                     double* abs_sum) {
+    const int NX = MODEL_CX, NY = MODEL_CY;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const long double lo = MODEL_XLO, hi = MODEL_XHI;
+    const long double mid = 0.5L * (lo + hi), half = 0.5L * (hi - lo);
+    long double f[NX][NY], tx[NX], ty[NY];
+    for (int a = 0; a < NX; ++a) tx[a] = cosl(pi * (a + 0.5L) / NX);
+    for (int b = 0; b < NY; ++b) ty[b] = cosl(pi * (b + 0.5L) / NY);
+    for (int a = 0; a < NX; ++a)
+        for (int b = 0; b < NY; ++b) f[a][b] = eval_monomial(coef, ex, nterm, mid + half * tx[a], mid + half * ty[b]);
+    double s = 0;
+    for (int p = 0; p < NX; ++p)
+        for (int q = 0; q < NY; ++q) {
+            long double acc = 0;
+            for (int a = 0; a < NX; ++a)
+                for (int b = 0; b < NY; ++b)
+                    acc += f[a][b] * cosl(p * pi * (a + 0.5L) / NX) * cosl(q * pi * (b + 0.5L) / NY);
+            acc *= (2.0L / NX) * (2.0L / NY);
+            if (p == 0) acc *= 0.5L;
+            if (q == 0) acc *= 0.5L;
+            out[p][q] = (float)acc;
+            s += fabsl(acc);
+        }
+    if (abs_sum) *abs_sum = s;
+}
+
+void model_cheb_tables(ModelCheb* m) {
+    cheb_fit(kB1, EXP_B1, 18, m->c[0], &m->abs_sum[0]);
+    cheb_fit(kC1, EXP_B1, 18, m->c[1], &m->abs_sum[1]);
+    cheb_fit(kB2, EXP_B2, 17, m->c[2], &m->abs_sum[2]);
+    cheb_fit(kC2, EXP_C2, 19, m->c[3], &m->abs_sum[3]);
+}
+
+// preProb of the full model (both logistic models, floor ln 9) at distance d,
+// x = min and y = max log size, in extended precision.
+static long double model_pre_ld(long double d, long double x, long double y) {
+    long double b = eval_monomial(kB1, EXP_B1, 18, x, y), c = eval_monomial(kC1, EXP_B1, 18, x, y);
+    long double pre = -(b + c * d);
+    if (pre <= (long double)MODEL_FLOOR_PREPROB) {
+        b = eval_monomial(kB2, EXP_B2, 17, x, y);
+        c = eval_monomial(kC2, EXP_C2, 19, x, y);
+        pre = -(b + c * d);
+        if (pre > (long double)MODEL_FLOOR_PREPROB) pre = MODEL_FLOOR_PREPROB;
+    }
+    return pre;
+}
+
+// Distance below which a pair of log sizes (x <= y) can clear preProb > T.
+// Both models decrease in d on the domain (c1 >= 19.6, c2 >= 15.5 there), so
+// the edge set of a length pair is {d < dthr}.  For T >= ln 9 only model 1
+// can pass: dthr = (-T - b1)/c1.  For T < ln 9, pre > T iff model 1 > ln 9 or
+// model 2 > T, so dthr = max((-ln9 - b1)/c1, (-T - b2)/c2).
+static long double dthr_ld(long double x, long double y, long double T) {
+    long double b1 = eval_monomial(kB1, EXP_B1, 18, x, y), c1 = eval_monomial(kC1, EXP_B1, 18, x, y);
+    if (T >= (long double)MODEL_FLOOR_PREPROB) return (-T - b1) / c1;
+    long double b2 = eval_monomial(kB2, EXP_B2, 17, x, y), c2 = eval_monomial(kC2, EXP_C2, 19, x, y);
+    long double d1 = (-(long double)MODEL_FLOOR_PREPROB - b1) / c1, d2 = (-T - b2) / c2;
+    return d1 > d2 ? d1 : d2;
+}
+
+// Upper bound of dthr over {xlo <= x <= y <= xhi}: the maximum on a 257 x 257
+// grid plus (grid step) x (largest finite-difference slope seen) as margin.
+double model_dthr_max(double xlo, double xhi, double T) {
+    const int G = 256;
+    long double lo = xlo, hi = xhi, h = (hi - lo) / G;
+    if (h <= 0) h = 1e-9L;
+    long double best = -1e30L, slope = 0;
+    std::vector<long double> v((G + 1) * (G + 1));
+    auto at = [&](int a, int b) -> long double& { return v[a * (G + 1) + b]; };
+    for (int a = 0; a <= G; ++a)
+        for (int b = a; b <= G; ++b) {
+            at(a, b) = dthr_ld(lo + h * a, lo + h * b, T);
+            if (at(a, b) > best) best = at(a, b);
+        }
+    for (int a = 0; a <= G; ++a)
+        for (int b = a; b <= G; ++b) {
+            if (b + 1 <= G) slope = fmaxl(slope, fabsl(at(a, b + 1) - at(a, b)) / h);
+            if (a + 1 <= b) slope = fmaxl(slope, fabsl(at(a + 1, b) - at(a, b)) / h);
+        }
+    return (double)(best + 2.0L * slope * h);
+}
+
+double model_pre_reference(double d, double x, double y) { return (double)model_pre_ld(d, x, y); }
+
+}  // namespace mdet
+
+extern "C" {
+
+// Chebyshev coefficients of b1, c1, b2, c2 (4 x MODEL_CX x MODEL_CY floats,
+// row p = degree in the min-size variable) for tests and the device upload.
+int mdet_model_cheb(float* out, double* abs_sum4, double* domain2) {
+    if (!out) MDET_FAIL(MDET_E_INVALID, "mdet_model_cheb: null output");
+    mdet::ModelCheb m;
+    mdet::model_cheb_tables(&m);
+    memcpy(out, m.c, sizeof m.c);
+    if (abs_sum4)
+        for (int i = 0; i < 4; ++i) abs_sum4[i] = m.abs_sum[i];
+    if (domain2) {
+        domain2[0] = MODEL_XLO;
+        domain2[1] = MODEL_XHI;
+    }
+    return MDET_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// Coefficient tables of the TNF pair model and the domain of the FP32
+// Chebyshev form.  Shared by host and device code.
+#pragma once
+
+// Domain of the log10 contig sizes accepted by the pair stage:
+// [log10 1500, log10 500000] (sizes are clamped at 500000, cuda/Cuda2.cu:1535,
+// and must be >= 1500, cuda/Cuda2.cu:1329-1332).
+#define MODEL_XLO 3.1760912590556813
+#define MODEL_XHI 5.698970004336019
+#define MODEL_CX 8 /* Chebyshev degrees 0..7 in x = min log size */
+#define MODEL_CY 7 /* Chebyshev degrees 0..6 in y = max log size */
+
+// ln 9: the floor of the first logistic model (cuda/Cuda2.cu:190).
+#define MODEL_FLOOR_PREPROB 2.1972245773362196
+
+namespace mdet {
+
+// Monomial coefficients of the two logistic models.  They come from upstream
+// MetaBAT2 via the reference (cuda/Cuda2.cu:170-188); the model is these
+// numbers, so they are the same values.  Term order and monomials:
+//   b1, c1: 1 x y x2 y2 x3 y3 x4 y4 x5 y5 x6 x7 xy x2y2 x3y3 x4y4 x5y5
+//   b2:     1 x y x2 y2 x3 y3 x4 y4 x5 y5 x6 y6 x2y2 x3y3 x4y4 x5y5
+//   c2:     1 x y x2 y2 x3 y3 x4 y4 x5 y5 x6 y6 x7 xy x2y2 x3y3 x4y4 x5y5
+// with x = min and y = max of the two log10 sizes.
+constexpr double kB1[18] = {46349.1624324381, -76092.3748553155, -639.918334183, 53873.3933743949,
+                            -156.6547554844, -21263.6010657275, 64.7719132839, 5003.2646455284,
+                            -8.5014386744, -700.5825500292, 0.3968284526, 54.037542743,
+                            -1.7713972342, 474.0850141891, -23.966597785, 0.7800219061,
+                            -0.0138723693, 0.0001027543};
+constexpr double kC1[18] = {-443565.465710869, 718862.10804858, 5114.1630934534, -501588.206183097,
+                            784.4442123743, 194712.394138513, -377.9645994741, -45088.7863182741,
+                            50.5960513287, 6220.3310639927, -2.3670776453, -473.269785487,
+                            15.3213264134, -3282.8510348085, 164.0438603974, -5.2778800755,
+                            0.0929379305, -0.0006826817};
+constexpr double kB2[17] = {6770.9351457442, -5933.7589419767, -2976.2879986855, 3279.7524685865,
+                            1602.7544794819, -967.2906583423, -462.0149190219, 159.8317289682,
+                            74.4884405822, -14.0267151808, -6.3644917671, 0.5108811613,
+                            0.2252455343, 0.965040193, -0.0546309127, 0.0012917084,
+                            -1.14383e-05};
+constexpr double kC2[19] = {39406.5712626297, -77863.1741143294, 9586.8761567725, 55360.1701572325,
+                            -5825.2491611377, -21887.8400068324, 1751.6803621934, 5158.3764225203,
+                            -290.1765894829, -724.0348081819, 25.364646181, 56.0522105105,
+                            -0.9172073892, -1.8470088417, 449.4660736502, -24.4141920625,
+                            0.8465834103, -0.0158943762, 0.0001235384};
+
+struct ModelCheb {
+    float c[4][MODEL_CX][MODEL_CY];  // b1, c1, b2, c2
+    double abs_sum[4];               // sum |C| of each expansion
+};
+void model_cheb_tables(ModelCheb* m);
+// Upper bound of the edge distance threshold over log sizes in [xlo, xhi].
+double model_dthr_max(double xlo, double xhi, double T);
+double model_pre_reference(double d, double x, double y);
+
+}  // namespace mdet

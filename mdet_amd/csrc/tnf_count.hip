@@ -1,0 +1,347 @@
+// TNF counting on gfx950: 2-bit packed stream -> per-contig canonical
+// tetranucleotide counts -> L2-normalised FP32 rows.
+//
+// Reference semantics: openmp/TNF.cpp:160-219 (oracle), cuda/Cuda2.cu:1823-1864
+// and cuda/Cuda.cu:222-269 (GPU variant: one thread walks a whole contig).
+//
+// Layout and work split (MI355X-first):
+//   * The packed stream (16 bases per uint32) is cut into fixed chunks of
+//     CNT_CHUNK bases of window END positions, one 256-thread block each, so
+//     every block streams the same number of bytes whatever the contig
+//     lengths (1.5 kb .. 2 Mb).  The 3-base halo of a window is read from the
+//     previous word, which the neighbouring lane loads anyway (L1/L2 hit).
+//   * Thread t reads words t, t+256, t+512, t+768 of the chunk: each load
+//     instruction of a wave covers 256 contiguous bytes.
+//   * Windows are binned by their raw 8-bit code into a block-shared LDS
+//     histogram per contig segment of the chunk; the fold to the 136 canonical
+//     columns (code and reverse complement) happens once per segment at flush,
+//     so the per-base work is extract + mask test + one LDS add.
+//   * N bases come as a sorted run list; the block looks up the runs that
+//     touch its chunk once and every thread ORs them into a per-word mask.
+#include <hipcub/hipcub.hpp>
+
+#include "hip_common.h"
+#include "tnf_tables.h"
+
+namespace {
+
+constexpr int CNT_THREADS = 256;
+constexpr int CNT_WORDS = 4;  // words per thread per chunk
+constexpr uint64_t CNT_CHUNK = (uint64_t)CNT_THREADS * CNT_WORDS * 16;
+constexpr int CNT_MAXSEG = 16;  // contig segments histogrammed per pass
+
+__constant__ int16_t c_colmap[256] = {
+#define M(i) mdet::kColumnMap.col[i]
+#define R8(i) M(i), M(i + 1), M(i + 2), M(i + 3), M(i + 4), M(i + 5), M(i + 6), M(i + 7)
+#define R64(i) R8(i), R8(i + 8), R8(i + 16), R8(i + 24), R8(i + 32), R8(i + 40), R8(i + 48), R8(i + 56)
+    R64(0), R64(64), R64(128), R64(192)
+#undef R64
+#undef R8
+#undef M
+};
+
+// upper_bound(a[0..len), v) - 1: last index with a[i] <= v, -1 if none.
+__device__ inline int64_t last_le(const uint64_t* a, int64_t len, uint64_t v) {
+    int64_t lo = 0, hi = len;  // answer in [lo-1, hi-1]
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo - 1;
+}
+
+// First run index k in [0, nr) whose end runs[2k+1] > v (ends are sorted).
+__device__ inline int64_t first_end_gt(const uint64_t* runs, int64_t nr, uint64_t v) {
+    int64_t lo = 0, hi = nr;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (runs[2 * mid + 1] > v) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+// First run index k whose start runs[2k] >= v.
+__device__ inline int64_t first_start_ge(const uint64_t* runs, int64_t nr, uint64_t v) {
+    int64_t lo = 0, hi = nr;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (runs[2 * mid] >= v) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __restrict__ packed, uint64_t n_bases,
+                                                              const uint64_t* __restrict__ runs, uint64_t n_runs,
+                                                              const uint64_t* __restrict__ base_off, uint32_t n,
+                                                              uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[CNT_MAXSEG][256];
+    __shared__ uint32_t fold[CNT_MAXSEG][136];
+    __shared__ uint64_t seg_lo[CNT_MAXSEG + 1];
+    __shared__ int64_t s_c[2];
+    __shared__ int64_t s_r[2];
+
+    const int tid = threadIdx.x;
+    const uint64_t chunk_lo = (uint64_t)blockIdx.x * CNT_CHUNK;
+    const uint64_t chunk_hi = min(chunk_lo + CNT_CHUNK, n_bases);
+
+    if (tid == 0) {
+        // contigs holding the first and last window end of the chunk
+        s_c[0] = last_le(base_off, (int64_t)n + 1, chunk_lo);
+        s_c[1] = last_le(base_off, (int64_t)n + 1, chunk_hi - 1);
+    } else if (tid == 64) {
+        uint64_t lo = chunk_lo >= 3 ? chunk_lo - 3 : 0;
+        s_r[0] = n_runs ? first_end_gt(runs, (int64_t)n_runs, lo) : 0;
+        s_r[1] = n_runs ? first_start_ge(runs, (int64_t)n_runs, chunk_hi) : 0;
+    }
+    __syncthreads();
+    int64_t c_first = s_c[0] < 0 ? 0 : s_c[0];
+    int64_t c_last = s_c[1] > (int64_t)n - 1 ? (int64_t)n - 1 : s_c[1];
+    const int64_t r_first = s_r[0], r_last = s_r[1];
+
+    // Per-word data is loaded once and kept for all passes.
+    uint64_t xw[CNT_WORDS];
+    uint32_t nmask[CNT_WORDS];  // bit j: base (16w - 3 + j) is N, j < 19
+    const uint64_t w0 = chunk_lo / 16;
+#pragma unroll
+    for (int it = 0; it < CNT_WORDS; ++it) {
+        const uint64_t w = w0 + tid + (uint64_t)it * CNT_THREADS;
+        const uint64_t p0 = w * 16;
+        xw[it] = 0;
+        nmask[it] = 0;
+        if (p0 < chunk_hi) {
+            uint32_t cur = packed[w];
+            uint32_t prev = w ? packed[w - 1] : 0u;
+            xw[it] = ((uint64_t)cur << 32) | prev;
+            const int64_t span_lo = (int64_t)p0 - 3, span_hi = (int64_t)p0 + 16;
+            for (int64_t r = r_first; r < r_last; ++r) {
+                int64_t rs = (int64_t)runs[2 * r], re = (int64_t)runs[2 * r + 1];
+                int64_t lo = rs > span_lo ? rs : span_lo;
+                int64_t hi = re < span_hi ? re : span_hi;
+                if (lo < hi) nmask[it] |= ((1u << (uint32_t)(hi - lo)) - 1u) << (uint32_t)(lo - span_lo);
+            }
+        }
+    }
+
+    for (int64_t cs = c_first; cs <= c_last; cs += CNT_MAXSEG) {
+        const int nseg = (int)min<int64_t>(CNT_MAXSEG, c_last - cs + 1);
+        if (tid <= nseg) seg_lo[tid] = base_off[cs + tid];
+        for (int i = tid; i < CNT_MAXSEG * 256; i += CNT_THREADS) (&hist[0][0])[i] = 0;
+        for (int i = tid; i < CNT_MAXSEG * 136; i += CNT_THREADS) (&fold[0][0])[i] = 0;
+        __syncthreads();
+        const uint64_t pass_lo = seg_lo[0], pass_hi = seg_lo[nseg];
+#pragma unroll
+        for (int it = 0; it < CNT_WORDS; ++it) {
+            const uint64_t p0 = (w0 + tid + (uint64_t)it * CNT_THREADS) * 16;
+            if (p0 >= chunk_hi || p0 + 16 <= pass_lo || p0 >= pass_hi) continue;
+            // segment of the word's first position, then walk forward
+            int s = 0;
+            while (s + 1 <= nseg && seg_lo[s + 1] <= p0) ++s;
+            const uint64_t x = xw[it];
+            const uint32_t nm = nmask[it];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const uint64_t p = p0 + k;
+                while (s < nseg && seg_lo[s + 1] <= p) ++s;
+                bool ok = (p >= chunk_lo) && (p < chunk_hi) && (p >= pass_lo) && (s < nseg) &&
+                          (p >= seg_lo[s] + 3) && (((nm >> k) & 0xFu) == 0);
+                if (ok) {
+                    uint32_t code = (uint32_t)(x >> (26 + 2 * k)) & 0xFFu;
+                    atomicAdd(&hist[s][code], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        // fold raw codes into canonical columns, then one global add per
+        // nonzero column (a contig may span several chunks)
+        for (int s = 0; s < nseg; ++s) {
+            uint32_t v = hist[s][tid];
+            if (v) atomicAdd(&fold[s][c_colmap[tid]], v);
+        }
+        __syncthreads();
+        for (int i = tid; i < nseg * 136; i += CNT_THREADS) {
+            int s = i / 136, col = i - s * 136;
+            uint32_t v = fold[s][col];
+            if (v) atomicAdd(&counts[(uint64_t)(cs + s) * 136 + col], v);
+        }
+        __syncthreads();
+    }
+}
+
+// One wavefront per contig: exact integer sum of squares, FP64 sqrt and
+// division (both correctly rounded), one rounding to FP32.  Equals
+// (float)(c / sqrt(sum c^2)) of openmp/TNF.cpp:208-217 bit for bit.
+__global__ __launch_bounds__(256) void k_normalize(const uint32_t* __restrict__ counts, uint32_t n,
+                                                   float* __restrict__ tnf) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const uint32_t* c = counts + r * 136;
+    uint32_t v0 = c[lane], v1 = c[lane + 64], v2 = lane < 8 ? c[lane + 128] : 0u;
+    uint64_t ss = (uint64_t)v0 * v0 + (uint64_t)v1 * v1 + (uint64_t)v2 * v2;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    const double rsum = sqrt((double)ss);
+    float* t = tnf + r * 136;
+    t[lane] = (float)((double)v0 / rsum);
+    t[lane + 64] = (float)((double)v1 / rsum);
+    if (lane < 8) t[lane + 128] = (float)((double)v2 / rsum);
+}
+
+// ---- raw-byte input (get_TNF's [begin | end] index, cuda/Cuda.cu:222-226) --
+// Contig c is bytes [index[c], index[n+c]).  Its windows (start positions
+// 0 .. len-4) are cut into chunks of ASC_CHUNK starts; one 256-thread block
+// per chunk stages the chunk's bytes (+3) in LDS, each thread forms 16 window
+// codes, and the block's 256-bin histogram is folded into the contig's 136
+// columns with one global add per nonzero column.
+constexpr int ASC_THREADS = 256;
+constexpr int ASC_CHUNK = ASC_THREADS * 16;
+
+__global__ void k_ascii_chunks(const uint64_t* __restrict__ index, uint32_t n, uint32_t* __restrict__ nch) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > n) return;
+    if (c == n) {
+        nch[n] = 0;
+        return;
+    }
+    const uint64_t b = index[c], e = index[n + c];
+    const uint64_t nw = e > b + 3 ? e - b - 3 : 0;  // window starts
+    nch[c] = (uint32_t)((nw + ASC_CHUNK - 1) / ASC_CHUNK);
+}
+
+__device__ inline uint32_t ascii_code(uint8_t ch) {
+    // A/a=0 C/c=1 G/g=2 T/t=3, anything else 4
+    switch (ch | 0x20) {
+        case 'a': return 0;
+        case 'c': return 1;
+        case 'g': return 2;
+        case 't': return 3;
+        default: return 4;
+    }
+}
+
+__global__ __launch_bounds__(ASC_THREADS) void k_count_ascii(const char* __restrict__ seqs,
+                                                             const uint64_t* __restrict__ index, uint32_t n,
+                                                             const uint64_t* __restrict__ choff,
+                                                             uint32_t* __restrict__ counts) {
+    __shared__ uint8_t code[ASC_CHUNK + 3];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t fold[136];
+    __shared__ int64_t s_c;
+    const int tid = threadIdx.x;
+    const uint64_t blk = blockIdx.x;
+    if (tid == 0) s_c = last_le(choff, (int64_t)n + 1, blk);
+    hist[tid] = 0;
+    if (tid < 136) fold[tid] = 0;
+    __syncthreads();
+    const int64_t c = s_c;  // choff[c] <= blk < choff[c+1] (empty contigs skipped)
+    const uint64_t b = index[c], e = index[n + c];
+    const uint64_t w0 = (blk - choff[c]) * (uint64_t)ASC_CHUNK;  // first window start of the chunk
+    const uint64_t nw = e - b - 3;
+    const uint32_t wn = (uint32_t)min<uint64_t>(ASC_CHUNK, nw - w0);
+    for (uint32_t k = tid; k < wn + 3; k += ASC_THREADS) code[k] = (uint8_t)ascii_code((uint8_t)seqs[b + w0 + k]);
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t w = tid * 16 + k;
+        if (w < wn) {
+            const uint32_t c0 = code[w], c1 = code[w + 1], c2 = code[w + 2], c3 = code[w + 3];
+            if ((c0 | c1 | c2 | c3) < 4) atomicAdd(&hist[c0 | (c1 << 2) | (c2 << 4) | (c3 << 6)], 1u);
+        }
+    }
+    __syncthreads();
+    if (hist[tid]) atomicAdd(&fold[c_colmap[tid]], hist[tid]);
+    __syncthreads();
+    if (tid < 136 && fold[tid]) atomicAdd(&counts[(uint64_t)c * 136 + tid], fold[tid]);
+}
+
+__global__ void k_logsize(const uint64_t* __restrict__ len, uint32_t n, double* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t v = len[i] < 500000u ? len[i] : 500000u;
+    out[i] = log10((double)v);
+}
+
+}  // namespace
+
+using namespace mdet;
+
+extern "C" {
+
+int mdet_tnf_count(mdet_ctx* ctx, const uint32_t* packed_dev, uint64_t n_bases, const uint64_t* runs_dev,
+                   uint64_t n_runs, const uint64_t* base_off_dev, uint32_t n, uint32_t* counts_dev,
+                   float* tnf_dev) {
+    MDET_TRY(check_ctx(ctx));
+    if (n == 0) return MDET_OK;
+    if (!base_off_dev || !tnf_dev) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_count: null base_off or tnf");
+    if (n_bases && !packed_dev) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_count: null packed stream");
+    if (n_runs && !runs_dev) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_count: n_runs > 0 with null runs");
+    if (n > (1u << 31)) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_count: n = %u contigs exceeds 2^31", n);
+    // The kernel reads only base_off[0..n], words below ceil(n_bases/16) and
+    // runs[0..2*n_runs); base_off beyond n_bases only drops windows.
+    uint64_t edge[2];
+    MDET_HIP(hipMemcpyAsync(&edge[0], base_off_dev, 8, hipMemcpyDeviceToHost, ctx->stream));
+    MDET_HIP(hipMemcpyAsync(&edge[1], base_off_dev + n, 8, hipMemcpyDeviceToHost, ctx->stream));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    if (edge[1] < edge[0]) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_count: base_off is not non-decreasing");
+    if (edge[1] > n_bases)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_count: base_off[n] = %llu exceeds n_bases = %llu",
+                  (unsigned long long)edge[1], (unsigned long long)n_bases);
+    uint32_t* counts = counts_dev;
+    if (!counts) MDET_TRY(ctx_buf(ctx, "count.counts", (size_t)n * 136, &counts));
+    MDET_HIP(hipMemsetAsync(counts, 0, (size_t)n * 136 * sizeof(uint32_t), ctx->stream));
+    const uint64_t nchunks = (n_bases + CNT_CHUNK - 1) / CNT_CHUNK;
+    if (nchunks > 0x7fffffffu) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_count: %llu bases is too many", (unsigned long long)n_bases);
+    if (nchunks)
+        hipLaunchKernelGGL(k_count_packed, dim3((unsigned)nchunks), dim3(CNT_THREADS), 0, ctx->stream, packed_dev,
+                           n_bases, runs_dev, n_runs, base_off_dev, n, counts);
+    MDET_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_normalize, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, counts, n, tnf_dev);
+    MDET_HIP(hipGetLastError());
+    return MDET_OK;
+}
+
+int mdet_tnf_count_ascii(mdet_ctx* ctx, const char* seqs_dev, const uint64_t* index_dev, uint32_t n,
+                         uint32_t* counts_dev, float* tnf_dev) {
+    MDET_TRY(check_ctx(ctx));
+    if (n == 0) return MDET_OK;
+    if (!seqs_dev || !index_dev || !tnf_dev) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_count_ascii: null argument");
+    if (n > (1u << 31)) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_count_ascii: n = %u contigs exceeds 2^31", n);
+    uint32_t* counts = counts_dev;
+    if (!counts) MDET_TRY(ctx_buf(ctx, "count.counts", (size_t)n * 136, &counts));
+    uint32_t* nch;
+    uint64_t* choff;
+    MDET_TRY(ctx_buf(ctx, "ascii.nch", (size_t)n + 1, &nch));
+    MDET_TRY(ctx_buf(ctx, "ascii.choff", (size_t)n + 1, &choff));
+    MDET_HIP(hipMemsetAsync(counts, 0, (size_t)n * 136 * sizeof(uint32_t), ctx->stream));
+    hipLaunchKernelGGL(k_ascii_chunks, dim3((n + 256) / 256), dim3(256), 0, ctx->stream, index_dev, n, nch);
+    MDET_HIP(hipGetLastError());
+    size_t tmp = 0;
+    MDET_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, nch, choff, (int)n + 1, ctx->stream));
+    void* t;
+    MDET_TRY(ctx_buf(ctx, "cub.tmp", tmp, &t));
+    MDET_HIP(hipcub::DeviceScan::ExclusiveSum(t, tmp, nch, choff, (int)n + 1, ctx->stream));
+    uint64_t total = 0;
+    MDET_HIP(hipMemcpyAsync(&total, choff + n, sizeof total, hipMemcpyDeviceToHost, ctx->stream));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    if (total > 0x7fffffffull) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_count_ascii: input too large");
+    if (total)
+        hipLaunchKernelGGL(k_count_ascii, dim3((unsigned)total), dim3(ASC_THREADS), 0, ctx->stream, seqs_dev,
+                           index_dev, n, choff, counts);
+    MDET_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_normalize, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, counts, n, tnf_dev);
+    MDET_HIP(hipGetLastError());
+    return MDET_OK;
+}
+
+int mdet_logsize(mdet_ctx* ctx, const uint64_t* lengths_dev, uint32_t n, double* out_dev) {
+    MDET_TRY(check_ctx(ctx));
+    if (n == 0) return MDET_OK;
+    if (!lengths_dev || !out_dev) MDET_FAIL(MDET_E_INVALID, "mdet_logsize: null argument");
+    hipLaunchKernelGGL(k_logsize, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, lengths_dev, n, out_dev);
+    MDET_HIP(hipGetLastError());
+    return MDET_OK;
+}
+
+}  // extern "C"

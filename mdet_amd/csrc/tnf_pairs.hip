@@ -1,0 +1,1060 @@
+// All-pairs TNF graph on gfx950: the thresholded, per-row-capped edge list of
+// gen_tnf_graph (cuda/Cuda2.cu:861-970) with the pair model of
+// cal_tnf_pre_dist_d (cuda/Cuda2.cu:192-241).
+//
+// Pipeline (one mdet_tnf_edges call):
+//   prep     per-contig: log-size -> Chebyshev coordinate, range check; for the
+//            MFMA modes the TNF rows in BF16/FP16, K padded 136 -> 144, stored
+//            in MFMA fragment order, and their FP32 squared norms.
+//   prefilter over the upper triangle of contig pairs, keep (i, j) whose
+//            distance can be below R = max over the length range of the edge
+//            distance threshold (+ the rounding error of the mode):
+//              EXACT_F32   FP32 sum of squared differences, VALU tiles
+//              *_REFINE    Gram d^2 = |a|^2 + |b|^2 - 2 a.b on MFMA
+//                          (v_mfma_f32_32x32x16_{bf16,f16}), FP32 accumulate
+//   refine   per survivor: FP32 sum of squared differences, FP32 model via
+//            the Chebyshev form; a pair whose FP32 preProb lies within EPS of
+//            the threshold (or whose model-1 value lies within EPS of ln 9,
+//            where the model switches) is decided by the FP64 tie-breaker,
+//            which repeats the reference's device arithmetic operation by
+//            operation.  Output: candidates (i, j, preProb).
+//   select   candidates -> per-row lists (both endpoints) -> per row the
+//            max_edges largest preProb (ties of FP32 values within 2*EPS of
+//            the cut are re-ranked in FP64) -> edges i < j, sorted.
+// The prefilter only has to be a superset, so every mode returns the same
+// edge set; the modes differ only in how much distance arithmetic is spent.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hip_common.h"
+#include "model_tables.h"
+
+namespace {
+
+constexpr int NT = 136;
+constexpr int KS = 9;  // MFMA k-steps of 16 (K padded to 144)
+constexpr float FLOOR32 = (float)MODEL_FLOOR_PREPROB;
+// Bound on |preProb_fp32 - preProb_fp64| used for the tie band.  Measured
+// errors of the FP32 evaluator are ~1e-5 (tests check < EPS/8).
+constexpr float EPS_PRE = 2e-3f;
+constexpr int SEL_CAP = 4096;  // candidates per row handled in LDS
+constexpr int ZONE_CAP = 512;  // ties re-ranked in FP64 per row
+
+__constant__ float c_cheb[4][MODEL_CX][MODEL_CY];
+
+#define E17(a) a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10], a[11], a[12], a[13], a[14], a[15], a[16]
+__constant__ double c_b1[18] = {E17(mdet::kB1), mdet::kB1[17]};
+__constant__ double c_c1[18] = {E17(mdet::kC1), mdet::kC1[17]};
+__constant__ double c_b2[17] = {E17(mdet::kB2)};
+__constant__ double c_c2[19] = {E17(mdet::kC2), mdet::kC2[17], mdet::kC2[18]};
+#undef E17
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+struct Cand {
+    uint32_t i, j;
+    float pre;
+    uint32_t flags;  // bit 0: preProb from the FP64 tie-breaker
+};
+struct Adj {
+    uint32_t partner;
+    float pre;
+};
+
+// ------------------------------------------------------------ FP32 model --
+__device__ inline float cheb_eval(int which, const float* Tx, const float* Ty) {
+    float acc = 0.f;
+#pragma unroll
+    for (int p = 0; p < MODEL_CX; ++p) {
+        float in = 0.f;
+#pragma unroll
+        for (int q = 0; q < MODEL_CY; ++q) in = fmaf(c_cheb[which][p][q], Ty[q], in);
+        acc = fmaf(in, Tx[p], acc);
+    }
+    return acc;
+}
+
+struct Pre32 {
+    float pre;   // full model
+    float pre1;  // model 1 alone (decides which model applies)
+};
+
+// preProb of a pair from its distance d and the Chebyshev coordinates of the
+// two log sizes.  t is increasing in the log size, so min/max commute.
+__device__ inline Pre32 model32(float d, float ta, float tb) {
+    const float tx = fminf(ta, tb), ty = fmaxf(ta, tb);
+    float Tx[MODEL_CX], Ty[MODEL_CY];
+    Tx[0] = 1.f;
+    Tx[1] = tx;
+#pragma unroll
+    for (int p = 2; p < MODEL_CX; ++p) Tx[p] = fmaf(2.f * tx, Tx[p - 1], -Tx[p - 2]);
+    Ty[0] = 1.f;
+    Ty[1] = ty;
+#pragma unroll
+    for (int q = 2; q < MODEL_CY; ++q) Ty[q] = fmaf(2.f * ty, Ty[q - 1], -Ty[q - 2]);
+    const float b1 = cheb_eval(0, Tx, Ty), c1 = cheb_eval(1, Tx, Ty);
+    Pre32 r;
+    r.pre1 = -fmaf(c1, d, b1);
+    r.pre = r.pre1;
+    if (r.pre1 <= FLOOR32) {
+        const float b2 = cheb_eval(2, Tx, Ty), c2 = cheb_eval(3, Tx, Ty);
+        r.pre = fminf(-fmaf(c2, d, b2), FLOOR32);
+    }
+    return r;
+}
+
+__device__ inline float dist2_f32(const float* __restrict__ a, const float* __restrict__ b) {
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* b4 = reinterpret_cast<const float4*>(b);
+    float s0 = 0.f, s1 = 0.f;
+#pragma unroll 2
+    for (int q = 0; q < NT / 4; ++q) {
+        float4 x = a4[q], y = b4[q];
+        float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
+        s0 = fmaf(d0, d0, s0);
+        s1 = fmaf(d1, d1, s1);
+        s0 = fmaf(d2, d2, s0);
+        s1 = fmaf(d3, d3, s1);
+    }
+    return s0 + s1;
+}
+
+// ------------------------------------------------------ FP64 tie-breaker --
+// Operation-for-operation the device arithmetic of the reference,
+// cuda/Cuda2.cu:192-241 compiled with fused multiply-add contraction:
+// float differences and squares accumulated in double, every "acc + c*m" of
+// the polynomials and "b + c*d" one fma, monomials as plain products.
+__device__ double model64(const float* __restrict__ a, const float* __restrict__ b, double r1, double r2) {
+#pragma clang fp contract(off)
+    double d = 0;
+    for (int i = 0; i < NT; ++i) {
+        float diff = a[i] - b[i];
+        float sq = diff * diff;
+        d += (double)sq;
+    }
+    d = sqrt(d);
+    double lw[19];
+    lw[0] = r1 < r2 ? r1 : r2;
+    lw[1] = r1 < r2 ? r2 : r1;
+    lw[2] = lw[0] * lw[0];
+    lw[4] = lw[2] * lw[0];
+    lw[6] = lw[4] * lw[0];
+    lw[8] = lw[6] * lw[0];
+    lw[10] = lw[8] * lw[0];
+    lw[11] = lw[10] * lw[0];
+    lw[3] = lw[1] * lw[1];
+    lw[5] = lw[3] * lw[1];
+    lw[7] = lw[5] * lw[1];
+    lw[9] = lw[7] * lw[1];
+    lw[12] = lw[0] * lw[1];
+    lw[14] = lw[4] * lw[5];
+    lw[15] = lw[6] * lw[7];
+    lw[16] = lw[8] * lw[9];
+    lw[13] = lw[2] * lw[3];
+    lw[18] = lw[9] * lw[1];
+    double bb = c_b1[0], cc = c_c1[0];
+#pragma unroll
+    for (int t = 1; t < 18; ++t) {
+        bb = __builtin_fma(c_b1[t], lw[t - 1], bb);
+        cc = __builtin_fma(c_c1[t], lw[t - 1], cc);
+    }
+    double pre = -__builtin_fma(cc, d, bb);
+    if (pre <= MODEL_FLOOR_PREPROB) {
+        // b2 terms use lw[0..10], lw[18], lw[13..16]; c2 adds lw[11], lw[12]
+        const int ib[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 18, 13, 14, 15, 16};
+        const int ic[18] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 18, 11, 12, 13, 14, 15, 16};
+        bb = c_b2[0];
+#pragma unroll
+        for (int t = 1; t < 17; ++t) bb = __builtin_fma(c_b2[t], lw[ib[t - 1]], bb);
+        cc = c_c2[0];
+#pragma unroll
+        for (int t = 1; t < 19; ++t) cc = __builtin_fma(c_c2[t], lw[ic[t - 1]], cc);
+        pre = -__builtin_fma(cc, d, bb);
+        if (pre > MODEL_FLOOR_PREPROB) pre = MODEL_FLOOR_PREPROB;
+    }
+    return pre;
+}
+
+// ------------------------------------------------------- wave utilities --
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// Reserve cnt slots per lane in a global list (all 64 lanes must call).
+// Returns this lane's first slot.
+__device__ inline uint64_t wave_reserve(uint32_t cnt, unsigned long long* counter) {
+    const uint32_t lane = lane_id();
+    uint32_t x = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    const uint32_t total = __shfl(x, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 63 && total) base = atomicAdd(counter, (unsigned long long)total);
+    base = __shfl(base, 63, 64);
+    return base + (x - cnt);
+}
+
+// --------------------------------------------------------------- prep ----
+// Chebyshev coordinate of each log size, domain check, min/max.
+__global__ void k_prep_rows(const double* __restrict__ ls, uint32_t n, float* __restrict__ tvec,
+                            unsigned long long* __restrict__ mm, unsigned int* __restrict__ bad) {
+    __shared__ unsigned long long smin[256], smax[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long lo = ~0ull, hi = 0ull;
+    if (i < n) {
+        const double v = ls[i];
+        // domain: [log10 1500, log10 500000] up to rounding of the caller's log10
+        if (!(v >= MODEL_XLO - 1e-12 && v <= MODEL_XHI + 1e-12)) atomicAdd(bad, 1u);
+        tvec[i] = (float)((2.0 * v - (MODEL_XLO + MODEL_XHI)) / (MODEL_XHI - MODEL_XLO));
+        if (v > 0) {
+            unsigned long long u = (unsigned long long)__double_as_longlong(v);
+            lo = u;
+            hi = u;
+        }
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < (unsigned)s) {
+            smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMin(&mm[0], smin[0]);
+        atomicMax(&mm[1], smax[0]);
+    }
+}
+
+// TNF transposed to K-major [136][n_pad] (zero padding) for the FP32 tiles.
+__global__ __launch_bounds__(256) void k_prep_f32t(const float* __restrict__ tnf, uint32_t n, uint32_t n_pad,
+                                                   float* __restrict__ tnft) {
+    __shared__ float tile[64][NT + 1];
+    const uint32_t r0 = blockIdx.x * 64;
+    for (int e = threadIdx.x; e < 64 * NT; e += 256) {
+        int r = e / NT, k = e - r * NT;
+        tile[r][k] = (r0 + r < n) ? tnf[(size_t)(r0 + r) * NT + k] : 0.f;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * NT; e += 256) {
+        int k = e / 64, r = e - k * 64;
+        tnft[(size_t)k * n_pad + r0 + r] = tile[r][k];
+    }
+}
+
+__device__ inline uint16_t to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Low-precision rows in MFMA fragment order.  For the 32-row tile t, k-step
+// ks and lane l (row t*32 + l%32, k = 16*ks + 8*(l/32) .. +8), the 16 bytes
+// sit at element offset ((t*9 + ks)*64 + l)*8: one coalesced 1 KiB load per
+// wave serves an A or B operand of v_mfma_f32_32x32x16.
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_prep_lowp(const float* __restrict__ tnf, uint32_t n, uint32_t n_pad,
+                                                   uint4* __restrict__ frag, float* __restrict__ norm) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n_pad) return;
+    float part = 0.f;
+    if (lane < 18) {
+        const int k0 = 8 * lane;
+        uint16_t h[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = k0 + e;
+            float v = (r < n && k < NT) ? tnf[(size_t)r * NT + k] : 0.f;
+            float back;
+            if (BF16) {
+                h[e] = to_bf16_rne(v);
+                back = from_bf16(h[e]);
+            } else {
+                _Float16 x = (_Float16)v;
+                h[e] = __builtin_bit_cast(uint16_t, x);
+                back = (float)x;
+            }
+            part = fmaf(back, back, part);
+        }
+        uint4 w;
+        w.x = h[0] | ((uint32_t)h[1] << 16);
+        w.y = h[2] | ((uint32_t)h[3] << 16);
+        w.z = h[4] | ((uint32_t)h[5] << 16);
+        w.w = h[6] | ((uint32_t)h[7] << 16);
+        const uint32_t t = r >> 5, ks = lane >> 1, half = lane & 1;
+        frag[((size_t)t * KS + ks) * 64 + (r & 31) + 32 * half] = w;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
+    if (lane == 0) norm[r] = (r < n) ? part : __builtin_inff();
+}
+
+// ------------------------------------------------------ FP32 prefilter ----
+// Upper-triangle 128x128 tiles; thread (ty, tx) owns rows ty+16a, columns
+// tx+16b (a, b < 8).  K streamed through LDS in chunks of 8 from the K-major
+// copy, so every global load is a contiguous 512-byte row segment.
+constexpr int F_T = 128, F_KC = 8;
+
+__global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__ tnft, uint32_t n, uint32_t n_pad,
+                                                       uint32_t ntiles, float R2, uint2* __restrict__ surv,
+                                                       unsigned long long cap, unsigned long long* __restrict__ counter) {
+    __shared__ float As[F_KC][F_T], Bs[F_KC][F_T];
+    // linear index -> (bi, bj), bi <= bj, rows of the triangle of length ntiles - bi
+    const uint64_t lin = blockIdx.x;
+    const double T = ntiles;
+    double disc = (2.0 * T + 1.0) * (2.0 * T + 1.0) - 8.0 * (double)lin;
+    int64_t bi = (int64_t)floor(((2.0 * T + 1.0) - sqrt(disc)) * 0.5);
+    auto start = [&](int64_t b) { return (uint64_t)(b * (int64_t)ntiles - b * (b - 1) / 2); };
+    while (bi > 0 && start(bi) > lin) --bi;
+    while (start(bi + 1) <= lin) ++bi;
+    const uint32_t bj = (uint32_t)(bi + (int64_t)(lin - start(bi)));
+    const uint32_t rb = (uint32_t)bi * F_T, cb = bj * F_T;
+
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    const int lk = tid >> 5, lc = (tid & 31) * 4;
+    float acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = 0.f;
+
+    float4 pa = *reinterpret_cast<const float4*>(tnft + (size_t)lk * n_pad + rb + lc);
+    float4 pb = *reinterpret_cast<const float4*>(tnft + (size_t)lk * n_pad + cb + lc);
+    for (int k0 = 0; k0 < NT; k0 += F_KC) {
+        *reinterpret_cast<float4*>(&As[lk][lc]) = pa;
+        *reinterpret_cast<float4*>(&Bs[lk][lc]) = pb;
+        __syncthreads();
+        if (k0 + F_KC < NT) {
+            pa = *reinterpret_cast<const float4*>(tnft + (size_t)(k0 + F_KC + lk) * n_pad + rb + lc);
+            pb = *reinterpret_cast<const float4*>(tnft + (size_t)(k0 + F_KC + lk) * n_pad + cb + lc);
+        }
+#pragma unroll
+        for (int kk = 0; kk < F_KC; ++kk) {
+            float av[8], bv[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) av[a] = As[kk][ty + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) bv[b] = Bs[kk][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const float df = av[a] - bv[b];
+                    acc[a][b] = fmaf(df, df, acc[a][b]);
+                }
+        }
+        __syncthreads();
+    }
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint32_t i = rb + ty + 16 * a, j = cb + tx + 16 * b;
+            cnt += (acc[a][b] < R2 && i < j && j < n) ? 1u : 0u;
+        }
+    if (__any(cnt != 0)) {
+        uint64_t pos = wave_reserve(cnt, counter);
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint32_t i = rb + ty + 16 * a, j = cb + tx + 16 * b;
+                if (acc[a][b] < R2 && i < j && j < n) {
+                    if (pos < cap) surv[pos] = make_uint2(i, j);
+                    ++pos;
+                }
+            }
+    }
+}
+
+// ------------------------------------------------------- MFMA prefilter ---
+// Block = 4 waves = 256 rows (64 per wave, two 32-row MFMA tiles held in
+// registers for the whole column sweep); columns stream through LDS in
+// steps of 128 (four 32-column tiles, 36 KiB in fragment order, loaded once
+// per block and read by all 4 waves).  Per step a wave issues 2 x 4 x 9
+// v_mfma_f32_32x32x16 into 8 independent accumulators, then tests
+//   d~^2 = n_i + n_j - 2 g < R2   <=>   g - n_j/2 > n_i/2 - R2/2
+// which is one subtract and one compare per pair.
+constexpr int M_ROWS = 256, M_STEP = 128, M_SEG = 2048;
+constexpr int M_STEP_U4 = M_STEP / 32 * KS * 64;  // uint4 per B step (2304)
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_prefilter_mfma(const uint4* __restrict__ frag, const float* __restrict__ norm,
+                                                        const uint4* __restrict__ items, uint32_t n, float R2,
+                                                        uint2* __restrict__ surv, unsigned long long cap,
+                                                        unsigned long long* __restrict__ counter) {
+    __shared__ uint4 Bl[M_STEP_U4];
+    const uint4 it = items[blockIdx.x];
+    const uint32_t rb = it.x, c_begin = it.y, c_end = it.z;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t row0 = rb * M_ROWS + wave * 64;  // this wave's 64 rows
+    const uint32_t t0 = row0 >> 5;
+
+    uint4 a[2][KS];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) a[rt][ks] = frag[((size_t)(t0 + rt) * KS + ks) * 64 + lane];
+    // row thresholds: accumulator r of row tile rt holds row
+    // row0 + 32*rt + 8*(r/4) + 4*(lane/32) + r%4, column lane%32
+    float ei[2][16];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t i = row0 + 32 * rt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            ei[rt][r] = fmaf(0.5f, norm[i], -0.5f * R2);
+        }
+
+    uint4 pf[KS];
+    const uint4* src = frag + (size_t)(c_begin / 32) * KS * 64;
+#pragma unroll
+    for (int m = 0; m < KS; ++m) pf[m] = src[tid + 256 * m];
+
+    for (uint32_t c0 = c_begin; c0 < c_end; c0 += M_STEP) {
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < KS; ++m) Bl[tid + 256 * m] = pf[m];
+        __syncthreads();
+        if (c0 + M_STEP < c_end) {
+            const uint4* nsrc = frag + (size_t)((c0 + M_STEP) / 32) * KS * 64;
+#pragma unroll
+            for (int m = 0; m < KS; ++m) pf[m] = nsrc[tid + 256 * m];
+        }
+        f32x16_t acc[2][4];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f32x16_t{};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            uint4 b[4];
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) b[ct] = Bl[(ct * KS + ks) * 64 + lane];
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) {
+                    if (BF16)
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8_t, a[rt][ks]), __builtin_bit_cast(bf16x8_t, b[ct]), acc[rt][ct],
+                            0, 0, 0);
+                    else
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            __builtin_bit_cast(f16x8_t, a[rt][ks]), __builtin_bit_cast(f16x8_t, b[ct]), acc[rt][ct],
+                            0, 0, 0);
+                }
+        }
+        float fj[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) fj[ct] = 0.5f * norm[c0 + 32 * ct + (lane & 31)];
+        bool hit = false;
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) hit |= (acc[rt][ct][r] - fj[ct]) > ei[rt][r];
+        if (__any(hit)) {
+            uint32_t cnt = 0;
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const uint32_t i = row0 + 32 * rt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                        const uint32_t j = c0 + 32 * ct + (lane & 31);
+                        cnt += ((acc[rt][ct][r] - fj[ct]) > ei[rt][r] && i < j) ? 1u : 0u;
+                    }
+            uint64_t pos = wave_reserve(cnt, counter);
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const uint32_t i = row0 + 32 * rt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                        const uint32_t j = c0 + 32 * ct + (lane & 31);
+                        if ((acc[rt][ct][r] - fj[ct]) > ei[rt][r] && i < j) {
+                            if (pos < cap) surv[pos] = make_uint2(i, j);
+                            ++pos;
+                        }
+                    }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- refine --
+__global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, uint64_t ns,
+                                                const float* __restrict__ tnf, const double* __restrict__ ls,
+                                                const float* __restrict__ tvec, double T, Cand* __restrict__ cand,
+                                                unsigned long long cap, unsigned long long* __restrict__ counters) {
+    const float T32 = (float)T;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x;
+    for (uint64_t base = first; base < ns; base += stride) {
+        const uint64_t idx = base + threadIdx.x;
+        bool keep = false, exact = false;
+        uint2 p = make_uint2(0, 0);
+        float key = 0.f;
+        if (idx < ns) {
+            p = surv[idx];
+            const float* a = tnf + (size_t)p.x * NT;
+            const float* b = tnf + (size_t)p.y * NT;
+            const float d = sqrtf(dist2_f32(a, b));
+            const Pre32 m = model32(d, tvec[p.x], tvec[p.y]);
+            const bool amb = fabsf(m.pre - T32) <= EPS_PRE || fabsf(m.pre1 - FLOOR32) <= EPS_PRE;
+            if (amb) {
+                const double p64 = model64(a, b, ls[p.x], ls[p.y]);
+                keep = p64 > T;
+                key = (float)p64;
+                exact = true;
+            } else {
+                keep = m.pre > T32;
+                key = m.pre;
+            }
+        }
+        const uint64_t nadj = __popcll(__ballot(exact));
+        if (lane_id() == 0 && nadj) atomicAdd(&counters[1], (unsigned long long)nadj);
+        uint64_t pos = wave_reserve(keep ? 1u : 0u, &counters[0]);
+        if (keep && pos < cap) {
+            Cand c;
+            c.i = p.x;
+            c.j = p.y;
+            c.pre = key;
+            c.flags = exact ? 1u : 0u;
+            cand[pos] = c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- select --
+__global__ void k_degree(const Cand* __restrict__ cand, uint64_t nc, uint32_t* __restrict__ deg) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    atomicAdd(&deg[cand[k].i], 1u);
+    atomicAdd(&deg[cand[k].j], 1u);
+}
+
+__global__ void k_scatter(const Cand* __restrict__ cand, uint64_t nc, const uint64_t* __restrict__ rowptr,
+                          uint32_t* __restrict__ cursor, Adj* __restrict__ adj) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nc) return;
+    const Cand c = cand[k];
+    adj[rowptr[c.i] + atomicAdd(&cursor[c.i], 1u)] = Adj{c.j, c.pre};
+    adj[rowptr[c.j] + atomicAdd(&cursor[c.j], 1u)] = Adj{c.i, c.pre};
+}
+
+// (pre desc, partner asc): true if x ranks before y
+__device__ inline bool ranks_before(float px, uint32_t ix, float py, uint32_t iy) {
+    return px > py || (px == py && ix < iy);
+}
+
+// Bitonic sort of sz (power of two) entries in LDS by ranks_before.
+__device__ void block_sort_rank(float* key, uint32_t* id, int sz) {
+    for (int k = 2; k <= sz; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < sz; t += blockDim.x) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const bool up = (t & k) == 0;
+                    const bool before = ranks_before(key[u], id[u], key[t], id[t]);
+                    if (before == up) {
+                        float tk = key[t];
+                        key[t] = key[u];
+                        key[u] = tk;
+                        uint32_t ti = id[t];
+                        id[t] = id[u];
+                        id[u] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Bitonic sort of sz entries by id ascending (key carried along).
+__device__ void block_sort_id(float* key, uint32_t* id, int sz) {
+    for (int k = 2; k <= sz; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < sz; t += blockDim.x) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const bool up = (t & k) == 0;
+                    if ((id[u] < id[t]) == up) {
+                        float tk = key[t];
+                        key[t] = key[u];
+                        key[u] = tk;
+                        uint32_t ti = id[t];
+                        id[t] = id[u];
+                        id[u] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// One block per row: pick the max_edges best partners (all if max_edges == 0
+// or fewer candidates), keep partners > row, sort them, write them back to the
+// front of the row's segment and their count to out_cnt[row].
+// err[0] |= 1: a row over SEL_CAP; err[0] |= 2: a tie zone over ZONE_CAP.
+constexpr int S_THREADS = 256;
+__global__ __launch_bounds__(S_THREADS) void k_select(const uint64_t* __restrict__ rowptr, Adj* __restrict__ adj,
+                                                      uint32_t n, uint32_t K, const float* __restrict__ tnf,
+                                                      const double* __restrict__ ls, uint32_t* __restrict__ out_cnt,
+                                                      unsigned int* __restrict__ err,
+                                                      unsigned long long* __restrict__ capped) {
+    __shared__ float key[SEL_CAP];
+    __shared__ uint32_t id[SEL_CAP];
+    __shared__ double zkey[ZONE_CAP];
+    __shared__ int s_z[2];
+    __shared__ uint32_t s_cnt;
+    const uint32_t row = blockIdx.x;
+    const uint64_t beg = rowptr[row];
+    const uint32_t deg = (uint32_t)(rowptr[row + 1] - beg);
+    if (deg == 0) {
+        if (threadIdx.x == 0) out_cnt[row] = 0;
+        return;
+    }
+    if (deg > SEL_CAP) {
+        if (threadIdx.x == 0) {
+            atomicOr(err, 1u);
+            out_cnt[row] = 0;
+        }
+        return;
+    }
+    int sz = 1;
+    while (sz < (int)deg) sz <<= 1;
+    for (int t = threadIdx.x; t < sz; t += blockDim.x) {
+        if (t < (int)deg) {
+            const Adj e = adj[beg + t];
+            key[t] = e.pre;
+            id[t] = e.partner;
+        } else {
+            key[t] = -__builtin_inff();
+            id[t] = 0xffffffffu;
+        }
+    }
+    __syncthreads();
+    uint32_t nsel = deg;
+    if (K && deg > K) {
+        if (threadIdx.x == 0) atomicAdd(capped, 1ull);
+        block_sort_rank(key, id, sz);
+        const float vk = key[K - 1];
+        // zone of FP32 keys within 2*EPS of the K-th: [z0, z1) in sorted order
+        if (threadIdx.x == 0) {
+            s_z[0] = (int)deg;
+            s_z[1] = 0;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < (int)deg; t += blockDim.x) {
+            const bool inz = fabsf(key[t] - vk) <= 2.f * EPS_PRE;
+            if (inz) {
+                atomicMin(&s_z[0], t);
+                atomicMax(&s_z[1], t + 1);
+            }
+        }
+        __syncthreads();
+        const int z0 = s_z[0], z1 = s_z[1], zn = z1 - z0;
+        if (zn > 1) {
+            if (zn > ZONE_CAP) {
+                if (threadIdx.x == 0) atomicOr(err, 2u);
+            } else {
+                const float* a = tnf + (size_t)row * NT;
+                for (int t = threadIdx.x; t < zn; t += blockDim.x) {
+                    const uint32_t p = id[z0 + t];
+                    zkey[t] = model64(a, tnf + (size_t)p * NT, ls[row], ls[p]);
+                }
+                __syncthreads();
+                // rank each zone entry by (FP64 key desc, partner asc)
+                uint32_t myid[2];
+                float mykey[2];
+                int myrank[2];
+                int cntm = 0;
+                for (int t = threadIdx.x; t < zn; t += blockDim.x, ++cntm) {
+                    int rk = 0;
+                    for (int u = 0; u < zn; ++u)
+                        rk += (zkey[u] > zkey[t] || (zkey[u] == zkey[t] && id[z0 + u] < id[z0 + t])) ? 1 : 0;
+                    myid[cntm] = id[z0 + t];
+                    mykey[cntm] = key[z0 + t];
+                    myrank[cntm] = rk;
+                }
+                __syncthreads();
+                for (int c = 0; c < cntm; ++c) {
+                    id[z0 + myrank[c]] = myid[c];
+                    key[z0 + myrank[c]] = mykey[c];
+                }
+                __syncthreads();
+            }
+        }
+        nsel = K;
+    }
+    // keep the selected partners > row, sorted ascending
+    for (int t = threadIdx.x; t < sz; t += blockDim.x)
+        if (t >= (int)nsel || id[t] <= row) id[t] = 0xffffffffu;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    block_sort_id(key, id, sz);
+    for (int t = threadIdx.x; t < sz; t += blockDim.x)
+        if (id[t] != 0xffffffffu) atomicAdd(&s_cnt, 1u);
+    __syncthreads();
+    const uint32_t cnt = s_cnt;
+    for (int t = threadIdx.x; t < (int)cnt; t += blockDim.x) adj[beg + t] = Adj{id[t], key[t]};
+    if (threadIdx.x == 0) out_cnt[row] = cnt;
+}
+
+__global__ void k_emit(const uint64_t* __restrict__ rowptr, const Adj* __restrict__ adj,
+                       const uint32_t* __restrict__ out_cnt, const uint64_t* __restrict__ eoff, uint32_t n,
+                       uint32_t* __restrict__ from, uint32_t* __restrict__ to, float* __restrict__ s) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const uint32_t c = out_cnt[row];
+    const uint64_t src = rowptr[row], dst = eoff[row];
+    for (uint32_t t = lane; t < c; t += 64) {
+        const Adj e = adj[src + t];
+        from[dst + t] = row;
+        to[dst + t] = e.partner;
+        s[dst + t] = 1.f - 1.f / (1.f + expf(e.pre));
+    }
+}
+
+__global__ void k_preprob_pairs(const float* __restrict__ tnf, const double* __restrict__ ls,
+                                const float* __restrict__ tvec, const uint32_t* __restrict__ pi,
+                                const uint32_t* __restrict__ pj, uint64_t np, float* __restrict__ p32,
+                                double* __restrict__ p64) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const float* a = tnf + (size_t)pi[k] * NT;
+    const float* b = tnf + (size_t)pj[k] * NT;
+    if (p32) p32[k] = model32(sqrtf(dist2_f32(a, b)), tvec[pi[k]], tvec[pj[k]]).pre;
+    if (p64) p64[k] = model64(a, b, ls[pi[k]], ls[pj[k]]);
+}
+
+}  // namespace
+
+namespace mdet {
+
+int ensure_model(mdet_ctx* ctx) {
+    if (ctx->model_ready) return MDET_OK;
+    ModelCheb m;
+    model_cheb_tables(&m);
+    MDET_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_cheb), m.c, sizeof m.c, 0, hipMemcpyHostToDevice, ctx->stream));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->model_ready = true;
+    return MDET_OK;
+}
+
+}  // namespace mdet
+
+using namespace mdet;
+
+namespace {
+
+struct Timer {
+    mdet_ctx* ctx;
+    int base;
+    int record(int k) {
+        hipEvent_t e;
+        MDET_TRY(ctx_event(ctx, base + k, &e));
+        MDET_HIP(hipEventRecord(e, ctx->stream));
+        return MDET_OK;
+    }
+    double ms(int a, int b) {
+        float v = 0;
+        if (hipEventElapsedTime(&v, ctx->events[base + a], ctx->events[base + b]) != hipSuccess) return 0;
+        return v;
+    }
+};
+
+template <class T>
+int read_scalar(mdet_ctx* ctx, const T* dev, T* out) {
+    void* st;
+    MDET_TRY(ctx_host_stage(ctx, 64, &st));
+    MDET_HIP(hipMemcpyAsync(st, dev, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(out, st, sizeof(T));
+    return MDET_OK;
+}
+
+int exclusive_scan_u32_to_u64(mdet_ctx* ctx, const uint32_t* in, uint64_t* out, uint32_t n) {
+    // out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total
+    size_t tmp = 0;
+    MDET_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n + 1, ctx->stream));
+    void* t;
+    MDET_TRY(ctx_buf(ctx, "cub.tmp", tmp, &t));
+    MDET_HIP(hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, (int)n + 1, ctx->stream));
+    return MDET_OK;
+}
+
+// Prefilter items for the MFMA kernel: (row block, column begin, column end).
+int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nitems) {
+    std::vector<uint4> items;
+    const uint32_t nrb = n_pad / M_ROWS;
+    for (uint32_t rb = 0; rb < nrb; ++rb)
+        for (uint32_t c = rb * M_ROWS; c < n_pad; c += M_SEG)
+            items.push_back(make_uint4(rb, c, std::min<uint32_t>(c + M_SEG, n_pad), 0));
+    *nitems = (uint32_t)items.size();
+    MDET_TRY(ctx_buf(ctx, "pairs.items", items.size(), items_dev));
+    MDET_HIP(hipMemcpyAsync(*items_dev, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                            ctx->stream));
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    return MDET_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
+                   uint32_t max_edges, int precision, mdet_edges* out, mdet_pair_stats* stats) {
+    MDET_TRY(check_ctx(ctx));
+    if (!out) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: null output");
+    *out = mdet_edges{0, nullptr, nullptr, nullptr};
+    mdet_pair_stats st{};
+    st.pairs_total = (uint64_t)n * (n ? n - 1 : 0) / 2;
+    if (!(cutoff > 0.0 && cutoff < 1.0)) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: cutoff %g not in (0, 1)", cutoff);
+    if (precision < MDET_EXACT_F32 || precision > MDET_F16_REFINE)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: unknown precision %d", precision);
+    if (n > (1u << 30)) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: n = %u too large", n);
+    if (n < 2) {
+        if (stats) *stats = st;
+        return MDET_OK;
+    }
+    if (!tnf_dev || !logsize_dev) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: null tnf or logsize");
+    MDET_TRY(ensure_model(ctx));
+    // threshold on preProb exactly as cuda/Cuda2.cu:884
+    const double T = log((1.0 / (1. - cutoff)) - 1.0);
+    Timer tm{ctx, 0};
+    MDET_TRY(tm.record(0));
+
+    // -- prep rows
+    float* tvec;
+    unsigned long long* mm;
+    MDET_TRY(ctx_buf(ctx, "pairs.tvec", n, &tvec));
+    MDET_TRY(ctx_buf(ctx, "pairs.mm", 8, &mm));
+    {
+        unsigned long long init[4] = {~0ull, 0ull, 0ull, 0ull};
+        void* st_host;
+        MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
+        memcpy(st_host, init, sizeof init);
+        MDET_HIP(hipMemcpyAsync(mm, st_host, sizeof init, hipMemcpyHostToDevice, ctx->stream));
+    }
+    unsigned int* bad = reinterpret_cast<unsigned int*>(mm + 2);
+    hipLaunchKernelGGL(k_prep_rows, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, logsize_dev, n, tvec, mm, bad);
+    MDET_HIP(hipGetLastError());
+    unsigned long long mmh[3];
+    {
+        void* st_host;
+        MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
+        MDET_HIP(hipMemcpyAsync(st_host, mm, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+        MDET_HIP(hipStreamSynchronize(ctx->stream));
+        memcpy(mmh, st_host, sizeof mmh);
+    }
+    if ((unsigned int)mmh[2] != 0)
+        MDET_FAIL(MDET_E_RANGE,
+                  "mdet_tnf_edges: %u log sizes outside [log10 1500, log10 500000] (contigs shorter than 1500 "
+                  "are not allowed in the graph)",
+                  (unsigned int)mmh[2]);
+    double xlo, xhi;
+    {
+        long long a = (long long)mmh[0], b = (long long)mmh[1];
+        memcpy(&xlo, &a, 8);
+        memcpy(&xhi, &b, 8);
+    }
+    const double R = model_dthr_max(xlo, xhi, T);
+    mdet_edges result{0, nullptr, nullptr, nullptr};
+
+    if (R > 0) {
+        // -- prefilter
+        const uint32_t n_pad = (n + M_ROWS - 1) / M_ROWS * M_ROWS;
+        unsigned long long* counters;  // [0] survivors [1] candidates [2] adjudicated
+        MDET_TRY(ctx_buf(ctx, "pairs.counters", 8, &counters));
+        uint64_t cap = std::max<uint64_t>(1u << 22, (uint64_t)n * 512);
+        auto it_cap = ctx->bufs.find("pairs.surv");
+        if (it_cap != ctx->bufs.end()) cap = std::max<uint64_t>(cap, it_cap->second.bytes / sizeof(uint2));
+        uint2* surv = nullptr;
+        uint64_t ns = 0;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            MDET_TRY(ctx_buf(ctx, "pairs.surv", cap, &surv));
+            MDET_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+            if (precision == MDET_EXACT_F32) {
+                float* tnft;
+                MDET_TRY(ctx_buf(ctx, "pairs.tnft", (size_t)NT * n_pad, &tnft));
+                hipLaunchKernelGGL(k_prep_f32t, dim3(n_pad / 64), dim3(256), 0, ctx->stream, tnf_dev, n, n_pad, tnft);
+                if (attempt == 0) MDET_TRY(tm.record(1));
+                const uint32_t ntiles = n_pad / F_T;
+                const uint64_t ntri = (uint64_t)ntiles * (ntiles + 1) / 2;
+                if (ntri > 0x7fffffffull) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: too many tiles");
+                const float R2 = (float)(R * R * (1.0 + 1e-4)) + 1e-6f;
+                hipLaunchKernelGGL(k_prefilter_f32, dim3((unsigned)ntri), dim3(256), 0, ctx->stream, tnft, n, n_pad,
+                                   ntiles, R2, surv, (unsigned long long)cap, counters);
+            } else {
+                const bool bf = precision == MDET_BF16_REFINE;
+                uint4* frag;
+                float* norm;
+                MDET_TRY(ctx_buf(ctx, "pairs.frag", (size_t)n_pad / 32 * KS * 64, &frag));
+                MDET_TRY(ctx_buf(ctx, "pairs.norm", n_pad, &norm));
+                if (bf)
+                    hipLaunchKernelGGL(k_prep_lowp<true>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
+                                       n_pad, frag, norm);
+                else
+                    hipLaunchKernelGGL(k_prep_lowp<false>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
+                                       n_pad, frag, norm);
+                uint4* items;
+                uint32_t nitems;
+                MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                if (attempt == 0) MDET_TRY(tm.record(1));
+                // |d~ - d| <= |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit
+                // roundoff of the format), plus FP32 accumulation slack
+                const double u = bf ? 1.0 / 256.0 : 1.0 / 2048.0;
+                const double Rm = R + 2.0 * u * 1.0001 + 2e-4;
+                const float R2 = (float)(Rm * Rm) + 1e-4f;
+                if (bf)
+                    hipLaunchKernelGGL(k_prefilter_mfma<true>, dim3(nitems), dim3(256), 0, ctx->stream, frag, norm,
+                                       items, n, R2, surv, (unsigned long long)cap, counters);
+                else
+                    hipLaunchKernelGGL(k_prefilter_mfma<false>, dim3(nitems), dim3(256), 0, ctx->stream, frag, norm,
+                                       items, n, R2, surv, (unsigned long long)cap, counters);
+            }
+            MDET_HIP(hipGetLastError());
+            if (attempt == 0) MDET_TRY(tm.record(2));
+            unsigned long long c;
+            MDET_TRY(read_scalar(ctx, counters, &c));
+            ns = c;
+            if (ns <= cap) break;
+            if (attempt == 1) MDET_FAIL(MDET_E_STATE, "mdet_tnf_edges: survivor count grew between passes");
+            cap = ns + ns / 8;
+        }
+        st.survivors = ns;
+        st.pairs_evaluated = st.pairs_total;
+
+        // -- refine
+        Cand* cand;
+        MDET_TRY(ctx_buf(ctx, "pairs.cand", std::max<uint64_t>(ns, 1), &cand));
+        unsigned long long* rc;
+        MDET_TRY(ctx_buf(ctx, "pairs.rcount", 4, &rc));
+        MDET_HIP(hipMemsetAsync(rc, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        if (ns) {
+            const uint64_t blocks = std::min<uint64_t>((ns + 255) / 256, (uint64_t)ctx->num_cus * 32);
+            hipLaunchKernelGGL(k_refine, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, surv, ns, tnf_dev,
+                               logsize_dev, tvec, T, cand, (unsigned long long)ns, rc);
+            MDET_HIP(hipGetLastError());
+        }
+        MDET_TRY(tm.record(3));
+        unsigned long long rch[2];
+        {
+            void* st_host;
+            MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
+            MDET_HIP(hipMemcpyAsync(st_host, rc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+            MDET_HIP(hipStreamSynchronize(ctx->stream));
+            memcpy(rch, st_host, sizeof rch);
+        }
+        const uint64_t nc = rch[0];
+        st.candidates = nc;
+        st.adjudicated = rch[1];
+
+        // -- select
+        uint32_t *deg, *cursor, *out_cnt;
+        uint64_t *rowptr, *eoff;
+        Adj* adj;
+        unsigned int* err;
+        unsigned long long* capped;
+        MDET_TRY(ctx_buf(ctx, "sel.deg", (size_t)n + 1, &deg));
+        MDET_TRY(ctx_buf(ctx, "sel.cursor", (size_t)n, &cursor));
+        MDET_TRY(ctx_buf(ctx, "sel.outcnt", (size_t)n + 1, &out_cnt));
+        MDET_TRY(ctx_buf(ctx, "sel.rowptr", (size_t)n + 1, &rowptr));
+        MDET_TRY(ctx_buf(ctx, "sel.eoff", (size_t)n + 1, &eoff));
+        MDET_TRY(ctx_buf(ctx, "sel.adj", std::max<uint64_t>(2 * nc, 1), &adj));
+        MDET_TRY(ctx_buf(ctx, "sel.err", 4, &err));
+        capped = reinterpret_cast<unsigned long long*>(err + 2);
+        MDET_HIP(hipMemsetAsync(deg, 0, ((size_t)n + 1) * 4, ctx->stream));
+        MDET_HIP(hipMemsetAsync(cursor, 0, (size_t)n * 4, ctx->stream));
+        MDET_HIP(hipMemsetAsync(out_cnt, 0, ((size_t)n + 1) * 4, ctx->stream));
+        MDET_HIP(hipMemsetAsync(err, 0, 16, ctx->stream));
+        if (nc) {
+            hipLaunchKernelGGL(k_degree, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream, cand, nc, deg);
+            MDET_TRY(exclusive_scan_u32_to_u64(ctx, deg, rowptr, n));
+            hipLaunchKernelGGL(k_scatter, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream, cand, nc,
+                               rowptr, cursor, adj);
+            hipLaunchKernelGGL(k_select, dim3(n), dim3(S_THREADS), 0, ctx->stream, rowptr, adj, n, max_edges, tnf_dev,
+                               logsize_dev, out_cnt, err, capped);
+            MDET_HIP(hipGetLastError());
+            MDET_TRY(exclusive_scan_u32_to_u64(ctx, out_cnt, eoff, n));
+            unsigned long long errh[2];
+            {
+                void* st_host;
+                MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
+                MDET_HIP(hipMemcpyAsync(st_host, err, 16, hipMemcpyDeviceToHost, ctx->stream));
+                MDET_HIP(hipStreamSynchronize(ctx->stream));
+                memcpy(errh, st_host, 16);
+            }
+            if ((unsigned int)errh[0] & 1u)
+                MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: a contig has more than %d candidate partners", SEL_CAP);
+            if ((unsigned int)errh[0] & 2u)
+                MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: more than %d near-tied candidates at a row's cut", ZONE_CAP);
+            st.rows_capped = errh[1];
+            uint64_t ne;
+            MDET_TRY(read_scalar(ctx, eoff + n, &ne));
+            uint32_t *from, *to;
+            float* s;
+            MDET_TRY(ctx_buf(ctx, "out.from", std::max<uint64_t>(ne, 1), &from));
+            MDET_TRY(ctx_buf(ctx, "out.to", std::max<uint64_t>(ne, 1), &to));
+            MDET_TRY(ctx_buf(ctx, "out.s", std::max<uint64_t>(ne, 1), &s));
+            hipLaunchKernelGGL(k_emit, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, rowptr, adj, out_cnt, eoff, n,
+                               from, to, s);
+            MDET_HIP(hipGetLastError());
+            result = mdet_edges{ne, from, to, s};
+        }
+    } else {
+        MDET_TRY(tm.record(1));
+        MDET_TRY(tm.record(2));
+        MDET_TRY(tm.record(3));
+    }
+    MDET_TRY(tm.record(4));
+    MDET_HIP(hipEventSynchronize(ctx->events[4]));
+    st.ms_prefilter = tm.ms(1, 2);
+    st.ms_refine = tm.ms(2, 3);
+    st.ms_select = tm.ms(3, 4);
+    st.ms_total = tm.ms(0, 4);
+    *out = result;
+    if (stats) *stats = st;
+    return MDET_OK;
+}
+
+int mdet_tnf_preprob(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n,
+                     const uint32_t* pi_dev, const uint32_t* pj_dev, uint64_t np, float* pre32_dev,
+                     double* pre64_dev) {
+    MDET_TRY(check_ctx(ctx));
+    if (np == 0) return MDET_OK;
+    if (!tnf_dev || !logsize_dev || !pi_dev || !pj_dev)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_preprob: null argument");
+    MDET_TRY(ensure_model(ctx));
+    float* tvec;
+    unsigned long long* mm;
+    MDET_TRY(ctx_buf(ctx, "pp.tvec", std::max<uint32_t>(n, 1), &tvec));
+    MDET_TRY(ctx_buf(ctx, "pp.mm", 8, &mm));
+    MDET_HIP(hipMemsetAsync(mm, 0, 64, ctx->stream));
+    hipLaunchKernelGGL(k_prep_rows, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, logsize_dev, n, tvec, mm,
+                       reinterpret_cast<unsigned int*>(mm + 2));
+    hipLaunchKernelGGL(k_preprob_pairs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, tnf_dev,
+                       logsize_dev, tvec, pi_dev, pj_dev, np, pre32_dev, pre64_dev);
+    MDET_HIP(hipGetLastError());
+    return MDET_OK;
+}
+
+}  // extern "C"

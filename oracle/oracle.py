@@ -1,0 +1,133 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+NTNF = 136
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = np.ctypeslib.ndpointer
+        L.oracle_count_seq.argtypes = [C.c_char_p, C.c_size_t, P(np.uint32, flags="C")]
+        L.oracle_count_ranges.argtypes = [C.c_char_p, P(np.uint64, flags="C"), P(np.uint64, flags="C"), C.c_size_t,
+                                          P(np.uint32, flags="C")]
+        L.oracle_normalize.argtypes = [P(np.uint32, flags="C"), C.c_size_t, P(np.float64, flags="C")]
+        L.oracle_normalize_f32.argtypes = [P(np.uint32, flags="C"), C.c_size_t, P(np.float32, flags="C")]
+        L.oracle_logsize.argtypes = [C.c_uint64]
+        L.oracle_logsize.restype = C.c_double
+        L.oracle_preprob_dev.argtypes = [P(np.float32, flags="C"), P(np.float32, flags="C"), C.c_double, C.c_double]
+        L.oracle_preprob_dev.restype = C.c_double
+        L.oracle_preprob_from_d.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.oracle_preprob_from_d.restype = C.c_double
+        L.oracle_tnf_dist_host.argtypes = [P(np.float32, flags="C"), P(np.float32, flags="C"), C.c_double, C.c_double]
+        L.oracle_tnf_dist_host.restype = C.c_double
+        L.oracle_model_coefs.argtypes = [C.c_double, C.c_double, P(np.float64, flags="C")]
+        L.oracle_cutoff_preprob.argtypes = [C.c_double]
+        L.oracle_cutoff_preprob.restype = C.c_double
+        L.oracle_coef_table.argtypes = [C.c_int, C.POINTER(C.c_int)]
+        L.oracle_coef_table.restype = C.POINTER(C.c_double)
+        L.oracle_coef_index.argtypes = [C.c_int]
+        L.oracle_coef_index.restype = C.POINTER(C.c_int)
+        L.oracle_tn_columns.restype = C.c_char_p
+        L.oracle_tnf_edges_rows.argtypes = [P(np.float32, flags="C"), P(np.float64, flags="C"), C.c_size_t, C.c_double,
+                                            C.c_size_t, C.c_size_t, C.c_size_t, P(np.uint32, flags="C"),
+                                            P(np.uint32, flags="C"), P(np.float64, flags="C"), C.c_size_t]
+        L.oracle_tnf_edges_rows.restype = C.c_size_t
+        L.oracle_rowmax.argtypes = [P(np.float32, flags="C"), P(np.float64, flags="C"), C.c_size_t,
+                                    P(np.uint64, flags="C"), C.c_size_t, P(np.float64, flags="C")]
+        _lib = L
+    return _lib
+
+
+def count_seq(seq: bytes) -> np.ndarray:
+    out = np.zeros(NTNF, np.uint32)
+    lib().oracle_count_seq(seq, len(seq), out)
+    return out
+
+
+def count_ranges(buf: bytes, begin: np.ndarray, end: np.ndarray) -> np.ndarray:
+    n = len(begin)
+    out = np.zeros((n, NTNF), np.uint32)
+    lib().oracle_count_ranges(buf, np.ascontiguousarray(begin, np.uint64), np.ascontiguousarray(end, np.uint64), n, out)
+    return out
+
+
+def normalize(counts: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros(c.shape, np.float64)
+    lib().oracle_normalize(c, c.shape[0], out)
+    return out
+
+
+def normalize_f32(counts: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros(c.shape, np.float32)
+    lib().oracle_normalize_f32(c, c.shape[0], out)
+    return out
+
+
+def logsize(lengths) -> np.ndarray:
+    return np.array([lib().oracle_logsize(int(v)) for v in lengths], np.float64)
+
+
+def preprob(a: np.ndarray, b: np.ndarray, r1: float, r2: float) -> float:
+    return lib().oracle_preprob_dev(np.ascontiguousarray(a, np.float32), np.ascontiguousarray(b, np.float32), r1, r2)
+
+
+def preprob_pairs(tnf: np.ndarray, ls: np.ndarray, pi, pj) -> np.ndarray:
+    L = lib()
+    tnf = np.ascontiguousarray(tnf, np.float32)
+    return np.array([L.oracle_preprob_dev(tnf[i], tnf[j], float(ls[i]), float(ls[j])) for i, j in zip(pi, pj)])
+
+
+def tnf_dist_host(a, b, r1, r2) -> float:
+    return lib().oracle_tnf_dist_host(np.ascontiguousarray(a, np.float32), np.ascontiguousarray(b, np.float32), r1, r2)
+
+
+def cutoff_preprob(cutoff: float) -> float:
+    return lib().oracle_cutoff_preprob(cutoff)
+
+
+def edges(tnf: np.ndarray, ls: np.ndarray, cutoff: float, max_edges: int, rows=None):
+    """(from, to, s) of gen_tnf_graph for rows [r0, r1) (all rows by default)."""
+    tnf = np.ascontiguousarray(tnf, np.float32)
+    ls = np.ascontiguousarray(ls, np.float64)
+    n = tnf.shape[0]
+    r0, r1 = (0, n) if rows is None else rows
+    cap = max(16, (r1 - r0) * (max_edges if max_edges else n))
+    f = np.zeros(cap, np.uint32)
+    t = np.zeros(cap, np.uint32)
+    s = np.zeros(cap, np.float64)
+    k = lib().oracle_tnf_edges_rows(tnf, ls, n, cutoff, max_edges, r0, r1, f, t, s, cap)
+    if k == (1 << 64) - 1:
+        raise RuntimeError("oracle edge buffer too small")
+    return f[:k].copy(), t[:k].copy(), s[:k].copy()
+
+
+def rowmax(tnf, ls, idx, n_sample) -> np.ndarray:
+    out = np.zeros(n_sample, np.float64)
+    lib().oracle_rowmax(np.ascontiguousarray(tnf, np.float32), np.ascontiguousarray(ls, np.float64), tnf.shape[0],
+                        np.ascontiguousarray(idx, np.uint64), n_sample, out)
+    return out

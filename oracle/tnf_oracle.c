@@ -1,0 +1,516 @@
+/*
+ * CPU oracle for the TNF contig-graph path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (mdet_amd/, include/)
+ * links, loads or calls this file.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg use it, as the checker and as the CPU baseline.
+ *
+ * Each function restates one piece of the reference algorithm in plain C and
+ * cites the reference lines it follows.  Parity pins (what checks each
+ * function against something other than itself):
+ *   oracle_count_seq      golden TNF.bin vectors written by the reference's own
+ *                         openmp/TNF.cpp (oracle/Makefile builds it), brute force
+ *                         over all 256 4-mers, reverse-complement invariance.
+ *   oracle_normalize      golden TNF.bin vectors (bit-exact FP64), closed form.
+ *   oracle_logsize        closed form log10(min(L, 500000)).
+ *   oracle_preprob_dev    exact rational evaluation of the coefficient tables in
+ *                         tests, cross-form agreement with oracle_tnf_dist_host
+ *                         and with the MetaBAT1 coefficient copy, closed forms
+ *                         (d = 0, model-2 clamp, symmetry).  Bit-for-bit
+ *                         agreement with the nvcc-compiled reference kernel is
+ *                         parity unpinned (the reference needs CUDA + Boost).
+ *   oracle_tnf_dist_host  same pins as oracle_preprob_dev.
+ *   oracle_tnf_edges      brute-force Python restatement on small inputs,
+ *                         invariants (threshold, per-row cap, i<j).
+ *   oracle_rowmax         brute-force restatement on small inputs.
+ *
+ * Build: oracle/Makefile -> oracle/liboracle.so (gcc -O2 -ffp-contract=off).
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NTNF 136
+
+/* Output column order of the 136 canonical tetranucleotides, as listed in
+ * openmp/TNF.cpp:14-24 (TN[]).  This order is the on-disk format of TNF.bin and
+ * of every TNF row, so it has to be identical. */
+static const char ORACLE_TN[NTNF * 4 + 1] =
+    "GGTAAGCCAAAAACATAGTCACGACATACGAAAAGTCAAACCAGGGACATTAGATCCCTCCTAAACTAAGGC"
+    "GCAACCGCCGCCAAACACTCATCCGACCGAGAATAGATCACAGAAGTAATGAAAATTTAATATAAGTGAGCT"
+    "CCACGGCCACCCGGGAGCGCATACCTGATAGAATATGTCACTCCACAAACCTTAAAAACGCGAGAGGGATCG"
+    "ACGCTCAACTACCTCAGACAGGAACTTCGCCCCTGCTGCAGGCACACGGAGCAACTCATGAATTACAGAGAT"
+    "ATAACATCGCCATCGACACACAACAAGGAGCAATGGATTCGTGAACCGGATAGCTACGTCCCCGAAGCCGTA"
+    "GTACAGGAAATGCACCCAGCCGGCACACCCGGCCGACCCCTGAAAACAAGAGCCCACGGATACAACCAACGT"
+    "GAACGTAAATGCGTTATCCACAGGACTGAAAGAAGACAAGGCGAAACCACGGCCAACTTAAGACAGCGGAAA"
+    "AATCATTGGCACCCTACGACCTAGAGAACGCACGCGAATA";
+
+/* The 16 reverse-complement palindromes, openmp/TNF.cpp:26-27 (TNP[]). */
+static const char ORACLE_TNP[16 * 4 + 1] =
+    "ACGTAGCTTCGATGCACATGCTAGGATCGTACATATTATACGCGGCGCAATTTTAACCGGGGCC";
+
+const char *oracle_tn_columns(void) { return ORACLE_TN; }
+
+/* Lookup "is this 4-char string in TN[] / TNP[]" the way the reference's
+ * unordered_map / unordered_set do (openmp/TNF.cpp:44-45,85-93): exact string
+ * match, so anything that is not four of 'A','C','G','T' is absent. */
+static int tn_find(const char w[4]) {
+    for (int i = 0; i < NTNF; ++i)
+        if (memcmp(ORACLE_TN + 4 * i, w, 4) == 0) return i;
+    return -1;
+}
+static int tnp_find(const char w[4]) {
+    for (int i = 0; i < 16; ++i)
+        if (memcmp(ORACLE_TNP + 4 * i, w, 4) == 0) return 1;
+    return 0;
+}
+
+/* openmp/TNF.cpp:47-63 revComp (applied to an already reversed window). */
+static int rev_comp_inplace(char *s, int size) {
+    for (int i = 0; i < size; ++i) {
+        if (s[i] == 'A') s[i] = 'T';
+        else if (s[i] == 'T') s[i] = 'A';
+        else if (s[i] == 'C') s[i] = 'G';
+        else if (s[i] == 'G') s[i] = 'C';
+        else return 0;
+    }
+    return 1;
+}
+
+/* Window -> column lookup table keyed by the four raw bytes would be faster,
+ * but the oracle follows the reference's per-window string logic literally;
+ * a 256^4 key space is avoided by a small cache on the 2-bit code when all
+ * four bytes are upper-case ACGT (same result, checked in tests by brute force
+ * over all 256 4-mers). */
+static int g_fwd[256], g_rc[256];
+static int g_tables_ready = 0;
+static int code_of(char c) {
+    switch (c) {
+        case 'A': return 0;
+        case 'C': return 1;
+        case 'G': return 2;
+        case 'T': return 3;
+        default: return -1;
+    }
+}
+static void build_tables(void) {
+    static const char B[4] = {'A', 'C', 'G', 'T'};
+    for (int code = 0; code < 256; ++code) {
+        char w[4];
+        for (int k = 0; k < 4; ++k) w[k] = B[(code >> (2 * (3 - k))) & 3];
+        g_fwd[code] = tn_find(w);
+        /* openmp/TNF.cpp:191-205: reverse, complement, count unless palindrome */
+        char r[4] = {w[3], w[2], w[1], w[0]};
+        rev_comp_inplace(r, 4);
+        g_rc[code] = tnp_find(r) ? -1 : tn_find(r);
+    }
+    g_tables_ready = 1;
+}
+
+/*
+ * Count the canonical tetranucleotides of one contig.
+ * Restates openmp/TNF.cpp:114 (toupper) and 174-206 (window loop): for every
+ * window of 4 bytes, +1 for the window itself if it is one of TN[], and +1 for
+ * its reverse complement if every byte is ACGT, it is not a palindrome and it
+ * is one of TN[].  Exactly one of the two lookups hits for a valid window.
+ */
+void oracle_count_seq(const char *seq, size_t len, uint32_t counts[NTNF]) {
+    if (!g_tables_ready) build_tables();
+    memset(counts, 0, NTNF * sizeof(uint32_t));
+    if (len < 4) return;
+    for (size_t i = 0; i + 3 < len; ++i) {
+        int code = 0, ok = 1;
+        for (int k = 0; k < 4; ++k) {
+            unsigned char ch = (unsigned char)seq[i + k];
+            if (ch >= 'a' && ch <= 'z') ch = (unsigned char)(ch - 'a' + 'A');
+            int c = code_of((char)ch);
+            if (c < 0) { ok = 0; break; }
+            code = (code << 2) | c;
+        }
+        if (!ok) continue; /* forward lookup misses, revComp fails -> continue */
+        if (g_fwd[code] >= 0) ++counts[g_fwd[code]];
+        if (g_rc[code] >= 0) ++counts[g_rc[code]];
+    }
+}
+
+/* Batch form over n contigs (seq pointers and lengths). */
+void oracle_count_batch(const char *const *seqs, const uint64_t *lens, size_t n, uint32_t *counts) {
+    for (size_t r = 0; r < n; ++r) oracle_count_seq(seqs[r], (size_t)lens[r], counts + r * NTNF);
+}
+
+/* Counts from one concatenated byte buffer with [begin[r], end[r]) ranges. */
+void oracle_count_ranges(const char *buf, const uint64_t *begin, const uint64_t *end, size_t n, uint32_t *counts) {
+    for (size_t r = 0; r < n; ++r)
+        oracle_count_seq(buf + begin[r], (size_t)(end[r] - begin[r]), counts + r * NTNF);
+}
+
+/* openmp/TNF.cpp:208-217: rsum = sqrt(sum c^2) in FP64, then c / rsum. */
+void oracle_normalize(const uint32_t *counts, size_t n, double *out) {
+    for (size_t r = 0; r < n; ++r) {
+        const uint32_t *c = counts + r * NTNF;
+        double rsum = 0;
+        for (int k = 0; k < NTNF; ++k) rsum += (double)c[k] * (double)c[k];
+        rsum = sqrt(rsum);
+        for (int k = 0; k < NTNF; ++k) out[r * NTNF + k] = (double)c[k] / rsum;
+    }
+}
+
+/* FP32 rows: the FP64 frequencies rounded once to float (the reference keeps
+ * TNF as float, cuda/Cuda2.cu:129,1852). */
+void oracle_normalize_f32(const uint32_t *counts, size_t n, float *out) {
+    double row[NTNF];
+    for (size_t r = 0; r < n; ++r) {
+        oracle_normalize(counts + r * NTNF, 1, row);
+        for (int k = 0; k < NTNF; ++k) out[r * NTNF + k] = (float)row[k];
+    }
+}
+
+/* cuda/Cuda2.cu:1535: logSizes = log10(min(size, 500000)). */
+double oracle_logsize(uint64_t len) {
+    uint64_t v = len < 500000u ? len : 500000u;
+    return log10((double)v);
+}
+
+/* Coefficient tables of the two logistic models.  They originate in upstream
+ * MetaBAT2 and appear in the reference at cuda/Cuda2.cu:170-188 (device arrays)
+ * and cuda/Cuda2.cu:747-771 (host expression); the model is defined by these
+ * numbers, so they are the same values.  Monomial order per entry below. */
+static const double B1[18] = {46349.1624324381, -76092.3748553155, -639.918334183, 53873.3933743949,
+                              -156.6547554844, -21263.6010657275, 64.7719132839, 5003.2646455284,
+                              -8.5014386744, -700.5825500292, 0.3968284526, 54.037542743,
+                              -1.7713972342, 474.0850141891, -23.966597785, 0.7800219061,
+                              -0.0138723693, 0.0001027543};
+static const double C1[18] = {-443565.465710869, 718862.10804858, 5114.1630934534, -501588.206183097,
+                              784.4442123743, 194712.394138513, -377.9645994741, -45088.7863182741,
+                              50.5960513287, 6220.3310639927, -2.3670776453, -473.269785487,
+                              15.3213264134, -3282.8510348085, 164.0438603974, -5.2778800755,
+                              0.0929379305, -0.0006826817};
+static const double B2[17] = {6770.9351457442, -5933.7589419767, -2976.2879986855, 3279.7524685865,
+                              1602.7544794819, -967.2906583423, -462.0149190219, 159.8317289682,
+                              74.4884405822, -14.0267151808, -6.3644917671, 0.5108811613,
+                              0.2252455343, 0.965040193, -0.0546309127, 0.0012917084,
+                              -1.14383e-05};
+static const double C2[19] = {39406.5712626297, -77863.1741143294, 9586.8761567725, 55360.1701572325,
+                              -5825.2491611377, -21887.8400068324, 1751.6803621934, 5158.3764225203,
+                              -290.1765894829, -724.0348081819, 25.364646181, 56.0522105105,
+                              -0.9172073892, -1.8470088417, 449.4660736502, -24.4141920625,
+                              0.8465834103, -0.0158943762, 0.0001235384};
+
+/* ln(9) = log(1/0.1 - 1): the floor of the first model, cuda/Cuda2.cu:190,722-724. */
+#define ORACLE_FLOOR_PREPROB 2.1972245773362196
+
+/* Squared Euclidean distance as the reference forms it: float differences,
+ * float squares, double accumulation in index order (cuda/Cuda2.cu:193-196). */
+static double ref_dist2(const float *a, const float *b) {
+    double d = 0;
+    for (int i = 0; i < NTNF; ++i) {
+        float diff = a[i] - b[i];
+        float sq = diff * diff;
+        d += (double)sq;
+    }
+    return d;
+}
+
+/* Monomials of the device form, indexed as cuda/Cuda2.cu:201-219 builds lw[]:
+ * lw[0]=x lw[1]=y lw[2]=x^2 lw[3]=y^2 lw[4]=x^3 lw[5]=y^3 lw[6]=x^4 lw[7]=y^4
+ * lw[8]=x^5 lw[9]=y^5 lw[10]=x^6 lw[11]=x^7 lw[12]=xy lw[13]=x^2y^2
+ * lw[14]=x^3y^3 lw[15]=x^4y^4 lw[16]=x^5y^5 lw[18]=y^6, x=min, y=max. */
+static void dev_monomials(double r1, double r2, double lw[19]) {
+    lw[0] = r1 < r2 ? r1 : r2;
+    lw[1] = r1 < r2 ? r2 : r1;
+    lw[2] = lw[0] * lw[0];
+    lw[4] = lw[2] * lw[0];
+    lw[6] = lw[4] * lw[0];
+    lw[8] = lw[6] * lw[0];
+    lw[10] = lw[8] * lw[0];
+    lw[11] = lw[10] * lw[0];
+    lw[3] = lw[1] * lw[1];
+    lw[5] = lw[3] * lw[1];
+    lw[7] = lw[5] * lw[1];
+    lw[9] = lw[7] * lw[1];
+    lw[12] = lw[0] * lw[1];
+    lw[14] = lw[4] * lw[5];
+    lw[15] = lw[6] * lw[7];
+    lw[16] = lw[8] * lw[9];
+    lw[13] = lw[2] * lw[3];
+    lw[18] = lw[9] * lw[1];
+}
+
+/* Which lw[] entry multiplies each coefficient (cuda/Cuda2.cu:221-236). */
+static const int IDX_B1[18] = {-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+static const int IDX_C1[18] = {-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+static const int IDX_B2[17] = {-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 18, 13, 14, 15, 16};
+static const int IDX_C2[19] = {-1, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 18, 11, 12, 13, 14, 15, 16};
+
+/* Left-to-right sum c0 + c1*m1 + ... with every "acc + c*m" contracted into
+ * one fused multiply-add, which is how nvcc (fmad=true) compiles the device
+ * expression at cuda/Cuda2.cu:221-236. */
+static double poly_fma(const double *coef, const int *idx, int nterm, const double lw[19]) {
+    double acc = coef[0];
+    for (int t = 1; t < nterm; ++t) acc = fma(coef[t], lw[idx[t]], acc);
+    return acc;
+}
+
+/*
+ * preProb of one pair as the reference's GPU path computes it:
+ * cuda/Cuda2.cu:192-241 (cal_tnf_pre_dist_d).  r1, r2 are log10 sizes.
+ * d = sqrt(sum (float diff)^2); model 1; if preProb <= ln 9, model 2 clamped
+ * to <= ln 9.  "b + c*d" is also a contracted fma in the device code.
+ */
+double oracle_preprob_dev(const float *a, const float *b, double r1, double r2) {
+    double d = sqrt(ref_dist2(a, b));
+    double lw[19];
+    dev_monomials(r1, r2, lw);
+    double bb = poly_fma(B1, IDX_B1, 18, lw);
+    double cc = poly_fma(C1, IDX_C1, 18, lw);
+    double pre = -fma(cc, d, bb);
+    if (pre <= ORACLE_FLOOR_PREPROB) {
+        bb = poly_fma(B2, IDX_B2, 17, lw);
+        cc = poly_fma(C2, IDX_C2, 19, lw);
+        pre = -fma(cc, d, bb);
+        if (pre > ORACLE_FLOOR_PREPROB) pre = ORACLE_FLOOR_PREPROB;
+    }
+    return pre;
+}
+
+/* Same, from a precomputed d (for tests that sweep d directly). */
+double oracle_preprob_from_d(double d, double r1, double r2) {
+    double lw[19];
+    dev_monomials(r1, r2, lw);
+    double bb = poly_fma(B1, IDX_B1, 18, lw);
+    double cc = poly_fma(C1, IDX_C1, 18, lw);
+    double pre = -fma(cc, d, bb);
+    if (pre <= ORACLE_FLOOR_PREPROB) {
+        bb = poly_fma(B2, IDX_B2, 17, lw);
+        cc = poly_fma(C2, IDX_C2, 19, lw);
+        pre = -fma(cc, d, bb);
+        if (pre > ORACLE_FLOOR_PREPROB) pre = ORACLE_FLOOR_PREPROB;
+    }
+    return pre;
+}
+
+/* The four model polynomials at (x=min, y=max) in the device form, for tests. */
+void oracle_model_coefs(double r1, double r2, double out[4]) {
+    double lw[19];
+    dev_monomials(r1, r2, lw);
+    out[0] = poly_fma(B1, IDX_B1, 18, lw);
+    out[1] = poly_fma(C1, IDX_C1, 18, lw);
+    out[2] = poly_fma(B2, IDX_B2, 17, lw);
+    out[3] = poly_fma(C2, IDX_C2, 19, lw);
+}
+
+/* Raw coefficient tables (for the exact-rational check in tests). */
+const double *oracle_coef_table(int which, int *nterm) {
+    switch (which) {
+        case 0: *nterm = 18; return B1;
+        case 1: *nterm = 18; return C1;
+        case 2: *nterm = 17; return B2;
+        default: *nterm = 19; return C2;
+    }
+}
+const int *oracle_coef_index(int which) {
+    switch (which) {
+        case 0: return IDX_B1;
+        case 1: return IDX_C1;
+        case 2: return IDX_B2;
+        default: return IDX_C2;
+    }
+}
+
+/*
+ * Host form cuda/Cuda2.cu:721-779 (cal_tnf_dist): same model written as named
+ * powers lw11..lw26, evaluated left to right with no contraction, returning the
+ * distance prob = 1/(1+exp(preProb)) with the 0.1 floor logic of lines 759-776.
+ */
+double oracle_tnf_dist_host(const float *a, const float *b, double ctg1, double ctg2) {
+    const double floor_prob = 0.1;
+    const double floor_pre = log((1.0 / floor_prob) - 1.0);
+    double d = sqrt(ref_dist2(a, b));
+    double lw11 = ctg1 < ctg2 ? ctg1 : ctg2;
+    double lw21 = ctg1 < ctg2 ? ctg2 : ctg1;
+    double lw12 = lw11 * lw11, lw13 = lw12 * lw11, lw14 = lw13 * lw11, lw15 = lw14 * lw11;
+    double lw16 = lw15 * lw11, lw17 = lw16 * lw11;
+    double lw22 = lw21 * lw21, lw23 = lw22 * lw21, lw24 = lw23 * lw21, lw25 = lw24 * lw21;
+    double lw26 = lw25 * lw21;
+    double bb = B1[0] + B1[1] * lw11 + B1[2] * lw21 + B1[3] * lw12 + B1[4] * lw22 + B1[5] * lw13 +
+                B1[6] * lw23 + B1[7] * lw14 + B1[8] * lw24 + B1[9] * lw15 + B1[10] * lw25 +
+                B1[11] * lw16 + B1[12] * lw17 + B1[13] * lw11 * lw21 + B1[14] * lw12 * lw22 +
+                B1[15] * lw13 * lw23 + B1[16] * lw14 * lw24 + B1[17] * lw15 * lw25;
+    double cc = C1[0] + C1[1] * lw11 + C1[2] * lw21 + C1[3] * lw12 + C1[4] * lw22 + C1[5] * lw13 +
+                C1[6] * lw23 + C1[7] * lw14 + C1[8] * lw24 + C1[9] * lw15 + C1[10] * lw25 +
+                C1[11] * lw16 + C1[12] * lw17 + C1[13] * lw11 * lw21 + C1[14] * lw12 * lw22 +
+                C1[15] * lw13 * lw23 + C1[16] * lw14 * lw24 + C1[17] * lw15 * lw25;
+    double pre = -(bb + cc * d);
+    double prob = pre <= floor_pre ? floor_prob : 1.0 / (1 + exp(pre));
+    if (prob >= floor_prob) {
+        bb = B2[0] + B2[1] * lw11 + B2[2] * lw21 + B2[3] * lw12 + B2[4] * lw22 + B2[5] * lw13 +
+             B2[6] * lw23 + B2[7] * lw14 + B2[8] * lw24 + B2[9] * lw15 + B2[10] * lw25 +
+             B2[11] * lw16 + B2[12] * lw26 + B2[13] * lw12 * lw22 + B2[14] * lw13 * lw23 +
+             B2[15] * lw14 * lw24 + B2[16] * lw15 * lw25;
+        cc = C2[0] + C2[1] * lw11 + C2[2] * lw21 + C2[3] * lw12 + C2[4] * lw22 + C2[5] * lw13 +
+             C2[6] * lw23 + C2[7] * lw14 + C2[8] * lw24 + C2[9] * lw15 + C2[10] * lw25 +
+             C2[11] * lw16 + C2[12] * lw26 + C2[13] * lw17 + C2[14] * lw11 * lw21 +
+             C2[15] * lw12 * lw22 + C2[16] * lw13 * lw23 + C2[17] * lw14 * lw24 +
+             C2[18] * lw15 * lw25;
+        pre = -(bb + cc * d);
+        prob = pre <= floor_pre ? 1.0 / (1 + exp(pre)) : floor_prob;
+    }
+    return prob;
+}
+
+/* Threshold on preProb for a similarity cutoff: cuda/Cuda2.cu:884. */
+double oracle_cutoff_preprob(double cutoff) { return log((1.0 / (1. - cutoff)) - 1.0); }
+
+/* --- std::priority_queue<pair<size_t,double>, vector, CompareEdge> ----------
+ * gen_tnf_graph keeps the best maxEdges per row in a min-heap ordered by
+ * CompareEdge (a.second > b.second, cuda/Cuda2.cu:660-664,896).  Which of two
+ * equal similarities is evicted depends on the heap algorithm, so the oracle
+ * runs the libstdc++ push_heap / pop_heap steps. */
+typedef struct { uint32_t j; double s; } heap_item;
+static int cmp_edge(const heap_item *a, const heap_item *b) { return a->s > b->s; }
+
+static void heap_sift_up(heap_item *h, size_t hole, size_t top, heap_item v) {
+    size_t parent = (hole - 1) / 2;
+    while (hole > top && cmp_edge(&h[parent], &v)) {
+        h[hole] = h[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    h[hole] = v;
+}
+static void heap_push(heap_item *h, size_t *len, heap_item v) {
+    h[*len] = v;
+    size_t n = ++*len;
+    if (n > 1) heap_sift_up(h, n - 1, 0, h[n - 1]);
+}
+static void heap_adjust(heap_item *h, size_t hole, size_t len, heap_item v) {
+    const size_t top = hole;
+    size_t second = hole;
+    while (len > 0 && second < (len - 1) / 2) {
+        second = 2 * (second + 1);
+        if (cmp_edge(&h[second], &h[second - 1])) second--;
+        h[hole] = h[second];
+        hole = second;
+    }
+    if ((len & 1) == 0 && len >= 2 && second == (len - 2) / 2) {
+        second = 2 * (second + 1);
+        h[hole] = h[second - 1];
+        hole = second - 1;
+    }
+    heap_sift_up(h, hole, top, v);
+}
+static void heap_pop(heap_item *h, size_t *len) {
+    size_t n = *len;
+    if (n > 1) {
+        heap_item v = h[n - 1];
+        h[n - 1] = h[0];
+        heap_adjust(h, 0, n - 1, v);
+    }
+    *len = n - 1;
+}
+
+/*
+ * One row of gen_tnf_graph (cuda/Cuda2.cu:896-946): every j != i in increasing
+ * order; similarity s = 1 - 1/(1+exp(preProb)) when preProb > cutoff threshold
+ * (cuda/Cuda2.cu:257-258); keep the best max_edges with the strict ">" rule of
+ * lines 928-931.  Writes the kept (j, s) into out_j/out_s (unordered) and
+ * returns how many; max_edges == 0 keeps every candidate.  heap must hold
+ * max(max_edges, n) items.
+ */
+size_t oracle_row_edges(const float *tnf, const double *logsize, size_t n, size_t i, double thr,
+                        size_t max_edges, heap_item *heap, uint32_t *out_j, double *out_s) {
+    size_t len = 0;
+    const float *a = tnf + i * NTNF;
+    for (size_t j = 0; j < n; ++j) {
+        if (j == i) continue;
+        double pre = oracle_preprob_dev(a, tnf + j * NTNF, logsize[i], logsize[j]);
+        if (!(pre > thr)) continue;
+        double s = 1. - 1. / (1. + exp(pre));
+        if (s == 0) continue; /* "if (sTNF && ...)" at line 928 */
+        heap_item it = {(uint32_t)j, s};
+        if (max_edges == 0 || len < max_edges) {
+            heap_push(heap, &len, it);
+        } else if (s > heap[0].s) {
+            heap_pop(heap, &len);
+            heap_push(heap, &len, it);
+        }
+    }
+    for (size_t k = 0; k < len; ++k) {
+        out_j[k] = heap[k].j;
+        out_s[k] = heap[k].s;
+    }
+    return len;
+}
+
+static int cmp_u32pair(const void *x, const void *y) {
+    const uint64_t a = *(const uint64_t *)x, b = *(const uint64_t *)y;
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+/*
+ * The edge list of gen_tnf_graph (cuda/Cuda2.cu:861-970) for rows
+ * [row_begin, row_end): for each row, its kept candidates with i < j
+ * (lines 936-943).  Output sorted by (i, j).  Returns the number of edges, or
+ * (size_t)-1 if cap is too small.  Rows are independent; threads > 1 splits
+ * them (only the CPU-baseline timing uses that).
+ */
+size_t oracle_tnf_edges_rows(const float *tnf, const double *logsize, size_t n, double cutoff,
+                             size_t max_edges, size_t row_begin, size_t row_end, uint32_t *from,
+                             uint32_t *to, double *s, size_t cap) {
+    double thr = oracle_cutoff_preprob(cutoff);
+    size_t hcap = max_edges ? max_edges : n;
+    heap_item *heap = (heap_item *)malloc(sizeof(heap_item) * (hcap + 1));
+    uint32_t *rj = (uint32_t *)malloc(sizeof(uint32_t) * (hcap + 1));
+    double *rs = (double *)malloc(sizeof(double) * (hcap + 1));
+    size_t cnt = 0;
+    int overflow = 0;
+    for (size_t i = row_begin; i < row_end && !overflow; ++i) {
+        size_t k = oracle_row_edges(tnf, logsize, n, i, thr, max_edges, heap, rj, rs);
+        /* sort this row's kept j ascending for a deterministic (i, j) order */
+        uint64_t *key = (uint64_t *)malloc(sizeof(uint64_t) * (k + 1));
+        for (size_t t = 0; t < k; ++t) key[t] = ((uint64_t)rj[t] << 32) | t;
+        qsort(key, k, sizeof(uint64_t), cmp_u32pair);
+        for (size_t t = 0; t < k; ++t) {
+            size_t src = (size_t)(key[t] & 0xffffffffu);
+            if (i < rj[src]) {
+                if (cnt >= cap) { overflow = 1; break; }
+                from[cnt] = (uint32_t)i;
+                to[cnt] = rj[src];
+                s[cnt] = rs[src];
+                ++cnt;
+            }
+        }
+        free(key);
+    }
+    free(heap);
+    free(rj);
+    free(rs);
+    return overflow ? (size_t)-1 : cnt;
+}
+
+size_t oracle_tnf_edges(const float *tnf, const double *logsize, size_t n, double cutoff,
+                        size_t max_edges, uint32_t *from, uint32_t *to, double *s, size_t cap) {
+    return oracle_tnf_edges_rows(tnf, logsize, n, cutoff, max_edges, 0, n, from, to, s, cap);
+}
+
+/*
+ * Sampled row maximum, cuda/Cuda2.cu:281-314 (get_tnf_max_prob_sample): for
+ * each sampled position p in [0, n_sample), the maximum preProb between
+ * contig idx[p] and every contig idx[q], q != p, q < n (the running maximum
+ * starts at DBL_MIN, line 287), mapped to 1 - 1/(1+exp(max)).
+ */
+void oracle_rowmax(const float *tnf, const double *logsize, size_t n, const uint64_t *idx,
+                   size_t n_sample, double *max_out) {
+    for (size_t p = 0; p < n_sample; ++p) {
+        double m = 2.2250738585072014e-308; /* DBL_MIN */
+        const uint64_t a = idx[p];
+        for (size_t q = 0; q < n; ++q) {
+            if (q == p) continue;
+            double v = oracle_preprob_dev(tnf + a * NTNF, tnf + idx[q] * NTNF, logsize[a], logsize[idx[q]]);
+            if (v > m) m = v;
+        }
+        max_out[p] = 1. - (1. / (1. + exp(m)));
+    }
+}

@@ -1,0 +1,116 @@
+"""CPU tests of the product's host logic: C ABI exports, FASTA reader, 2-bit
+packer, synthetic generator.  No device calls."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mdet_amd as M
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "mdet", "tnf.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mdet_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    lib = ctypes.CDLL(M.LIB_PATH)
+    decl = declared_symbols()
+    assert len(decl) >= 20
+    missing = [s for s in decl if not hasattr(lib, s)]
+    assert not missing, missing
+    # the Python binding covers the same surface
+    assert set(M.exported_symbols()) <= set(decl)
+
+
+def test_version_and_error_channel():
+    assert b"gfx950" in M.lib().mdet_version()
+    with pytest.raises(M.MdetError, match="null"):
+        M._check(M.lib().mdet_fasta_scan(None, 10, None, None), "scan")
+
+
+# ------------------------------------------------------------------ FASTA --
+def test_fasta_reader_kseq_semantics():
+    data = (b"junk before\n>a desc here\r\nACGT\r\nac\r\n\r\n>b\nAC\n\nGT\n>\nNNN\n>c\tx\n\r\nAC\r\n"
+            b">d\nA\n\r")
+    recs = M.fasta_read(data)
+    names = [n for n, _ in recs]
+    seqs = [s for _, s in recs]
+    assert names == [b"a", b"b", b"", b"c", b"d"]
+    # CRLF stripped per line; blank lines skipped; a '\r'-only first line is
+    # kept (accumulated length 1); a final 1-byte line without newline keeps it
+    assert seqs[0] == b"ACGTac"
+    assert seqs[1] == b"ACGT"
+    assert seqs[2] == b"NNN"
+    assert seqs[3] == b"\rAC"
+    assert seqs[4] == b"A\r"
+
+
+def test_fasta_rejects_fastq():
+    with pytest.raises(M.MdetError, match="FASTQ"):
+        M.fasta_read(b"@r1\nACGT\n+\nIIII\n")
+
+
+def test_fasta_fixture_records(golden):
+    recs = golden["records"]
+    assert len(recs) == golden["meta"]["n_records"]
+    assert all(b"\r" not in s and b"\n" not in s for _, s in recs)
+
+
+# ---------------------------------------------------------------- packing --
+def _unpack(packed, runs, off, r):
+    a, b = int(off[r]), int(off[r + 1])
+    out = bytearray()
+    for p in range(a, b):
+        out.append(b"ACGT"[(int(packed[p // 16]) >> (2 * (p % 16))) & 3])
+    for k in range(0, len(runs), 2):
+        s, e = int(runs[k]), int(runs[k + 1])
+        for p in range(max(s, a), min(e, b)):
+            out[p - a] = ord("N")
+    return bytes(out)
+
+
+def test_pack_roundtrip_and_runs():
+    rng = np.random.default_rng(1)
+    seqs = [bytes(rng.choice(list(b"ACGTacgtNRY"), size=int(rng.integers(0, 90)))) for _ in range(40)]
+    packed, runs, off = M.pack_ascii(seqs)
+    assert np.all(runs[1::2] > runs[0::2]) and np.all(runs[2::2] > runs[1:-1:2])  # sorted, disjoint
+    for r, s in enumerate(seqs):
+        want = bytes(c if c in b"ACGT" else ord("N") for c in s.upper())
+        assert _unpack(packed, runs, off, r) == want
+
+
+# -------------------------------------------------------------- generator --
+def test_generator_deterministic_and_thread_independent():
+    cfg = M.gen_config(seed=5, n_genomes=7, len_median=3000, len_min=1000, len_max=40000, n_frac=0.02)
+    p1, r1, o1, l1, g1 = M.gen_packed(cfg, 300, threads=1)
+    p8, r8, o8, l8, g8 = M.gen_packed(cfg, 300, threads=8)
+    assert np.array_equal(p1, p8) and np.array_equal(r1, r8) and np.array_equal(l1, l8)
+    seqs = [M.gen_ascii(cfg, i, int(l1[i])) for i in range(300)]
+    pa, ra, oa = M.pack_ascii(seqs)
+    assert np.array_equal(pa, p1) and np.array_equal(ra, r1) and np.array_equal(oa, o1)
+    assert np.all(l1 >= 1000) and np.all(l1 <= 40000)
+    nfrac = sum(s.count(b"N") for s in seqs) / sum(len(s) for s in seqs)
+    assert 0.01 < nfrac < 0.03
+    assert len(set(g1.tolist())) == 7
+
+
+def test_generator_genomes_have_distinct_composition():
+    cfg = M.gen_config(seed=9, n_genomes=4, len_median=20000, len_sigma=0.1, len_min=15000, len_max=30000)
+    lens, gid = M.gen_lengths(cfg, 40)
+    tnf = O.normalize_f32(np.stack([O.count_seq(M.gen_ascii(cfg, i, int(lens[i]))) for i in range(40)]))
+    d = np.linalg.norm(tnf[:, None, :] - tnf[None, :, :], axis=2)
+    same = gid[:, None] == gid[None, :]
+    off = ~np.eye(40, dtype=bool)
+    assert d[same & off].max() < d[~same].min()
+
+
+def test_generator_rejects_bad_config():
+    with pytest.raises(M.MdetError):
+        M.gen_lengths(M.gen_config(n_genomes=0), 3)

@@ -1,0 +1,254 @@
+"""CPU checks of the oracle itself: golden vectors written by the reference's
+own openmp/TNF.cpp, brute force over small inputs, closed forms and
+invariants.  These pin the oracle before it is used to judge the GPU path."""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import mdet_amd as M
+from oracle import oracle as O
+
+COMP = {"A": "T", "C": "G", "G": "C", "T": "A"}
+
+
+def revcomp(s: str) -> str:
+    return "".join(COMP[c] for c in reversed(s))
+
+
+def all_kmers(k=4):
+    out = [""]
+    for _ in range(k):
+        out = [p + b for p in out for b in "ACGT"]
+    return out
+
+
+# ----------------------------------------------------------------- columns --
+def test_column_tables_agree_and_are_canonical():
+    cols_oracle = [O.lib().oracle_tn_columns().decode()[4 * i:4 * i + 4] for i in range(136)]
+    cols_prod = M.tnf_columns()
+    assert cols_oracle == cols_prod
+    assert len(set(cols_prod)) == 136
+    # every column is the lexicographic minimum of its reverse-complement class,
+    # and the 136 columns are exactly the 136 classes of the 256 4-mers
+    classes = {min(w, revcomp(w)) for w in all_kmers()}
+    assert set(cols_prod) == classes
+    assert sum(1 for c in cols_prod if c == revcomp(c)) == 16
+
+
+def test_product_column_of_code_matches_definition():
+    col = M.column_of_code()
+    cols = M.tnf_columns()
+    idx = {c: i for i, c in enumerate(cols)}
+    for code in range(256):
+        w = "".join("ACGT"[(code >> (2 * k)) & 3] for k in range(4))  # first base in the low bits
+        assert col[code] == idx[min(w, revcomp(w))]
+
+
+# ---------------------------------------------------------------- counting --
+def test_oracle_counts_match_reference_golden_vectors(golden):
+    """openmp/TNF.cpp's TNF.bin on the fixture == oracle, bit for bit (FP64)."""
+    recs, meta, ref = golden["records"], golden["meta"], golden["tnf"]
+    assert ref.shape[0] == len(meta["rows"])
+    for r, k in enumerate(meta["rows"]):
+        seq = recs[k][1]
+        assert len(seq) == meta["lengths"][r]
+        if len(seq) < 2500:  # TNF.cpp leaves rows of [1000, 2500) at zero
+            assert not ref[r].any()
+            continue
+        got = O.normalize(O.count_seq(seq)[None, :])[0]
+        assert np.array_equal(got.view(np.uint64), ref[r].view(np.uint64)), f"row {r}"
+
+
+def test_golden_fixture_covers_the_hard_cases(golden):
+    fa = golden["fasta"]
+    assert b"\r\n" in fa and b"\n\n" in fa
+    seqs = b"".join(s for _, s in golden["records"])
+    assert any(c in seqs for c in b"acgt") and b"N" in seqs and any(c in seqs for c in b"RYKMSW")
+    lens = [len(s) for _, s in golden["records"]]
+    assert min(lens) < 1000 and any(1000 <= v < 2500 for v in lens) and max(lens) > 10000
+
+
+def test_oracle_bruteforce_single_kmers():
+    cols = M.tnf_columns()
+    idx = {c: i for i, c in enumerate(cols)}
+    for w in all_kmers():
+        for case in (w, w.lower()):
+            c = O.count_seq(case.encode())
+            assert c.sum() == 1 and c[idx[min(w, revcomp(w))]] == 1
+
+
+def test_oracle_invalid_bytes_void_windows():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        s = bytearray(rng.choice(list(b"ACGTacgtNRY-\r"), size=int(rng.integers(0, 60))))
+        c = O.count_seq(bytes(s))
+        valid = 0
+        up = bytes(s).upper()
+        for i in range(len(s) - 3):
+            valid += all(ch in b"ACGT" for ch in up[i:i + 4])
+        assert c.sum() == valid
+
+
+def test_oracle_reverse_complement_invariance():
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        s = "".join(rng.choice(list("ACGTN"), p=[.24, .24, .24, .24, .04], size=int(rng.integers(4, 3000))))
+        rc = "".join({"A": "T", "C": "G", "G": "C", "T": "A", "N": "N"}[c] for c in reversed(s))
+        assert np.array_equal(O.count_seq(s.encode()), O.count_seq(rc.encode()))
+
+
+def test_oracle_normalize_closed_form():
+    rng = np.random.default_rng(7)
+    c = rng.integers(0, 5000, size=(40, 136)).astype(np.uint32)
+    c[3] = 0
+    got = O.normalize(c)
+    with np.errstate(invalid="ignore"):
+        want = c / np.sqrt((c.astype(np.float64) ** 2).sum(1, keepdims=True))
+    assert np.array_equal(got[~np.isnan(want)], want[~np.isnan(want)])
+    assert np.isnan(got[3]).all()  # 0/0, as openmp/TNF.cpp:213-216
+    f32 = O.normalize_f32(c)
+    assert np.array_equal(f32[:3], want[:3].astype(np.float32))
+
+
+def test_logsize_closed_form():
+    for L in [1, 1500, 2500, 123456, 499999, 500000, 500001, 2_000_000]:
+        assert O.lib().oracle_logsize(L) == math.log10(min(L, 500000))
+    lens = np.array([1500, 7777, 500000, 3_000_000], np.uint64)
+    assert np.array_equal(M.logsize_host(lens), O.logsize(lens))
+
+
+# ------------------------------------------------------------------- model --
+def _exact_poly(coef, exps, x, y):
+    X, Y = Fraction(x), Fraction(y)
+    return sum(Fraction(c) * X ** a * Y ** b for c, (a, b) in zip(coef, exps))
+
+
+def _tables():
+    import ctypes
+    out = []
+    for w in range(4):
+        n = ctypes.c_int(0)
+        p = O.lib().oracle_coef_table(w, ctypes.byref(n))
+        out.append([p[k] for k in range(n.value)])
+    return out
+
+
+# Exponents of the device-form monomials lw[0..18] (cuda/Cuda2.cu:201-219)
+LW_EXP = {0: (1, 0), 1: (0, 1), 2: (2, 0), 3: (0, 2), 4: (3, 0), 5: (0, 3), 6: (4, 0), 7: (0, 4), 8: (5, 0),
+          9: (0, 5), 10: (6, 0), 11: (7, 0), 12: (1, 1), 13: (2, 2), 14: (3, 3), 15: (4, 4), 16: (5, 5), 18: (0, 6)}
+
+
+def test_model_device_form_vs_exact_rational():
+    """The oracle's FP64 fma evaluation of b1 c1 b2 c2 against exact rational
+    evaluation of the same tables (index map of the device form)."""
+    tabs = _tables()
+    rng = np.random.default_rng(11)
+    for _ in range(60):
+        x, y = sorted(rng.uniform(math.log10(1500), math.log10(500000), 2))
+        got = np.zeros(4)
+        O.lib().oracle_model_coefs(float(x), float(y), got)
+        for w in range(4):
+            idx = O.lib().oracle_coef_index(w)
+            exps = [(0, 0)] + [LW_EXP[idx[t]] for t in range(1, len(tabs[w]))]
+            exact = _exact_poly(tabs[w], exps, float(x), float(y))
+            scale = sum(abs(Fraction(c) * Fraction(float(x)) ** a * Fraction(float(y)) ** b)
+                        for c, (a, b) in zip(tabs[w], exps))
+            assert abs(float(exact) - got[w]) <= 40 * 2.2e-16 * float(scale), (w, x, y)
+
+
+def test_model_three_transcriptions_agree():
+    """Device form (oracle), host form (oracle, cuda/Cuda2.cu:747-771) and the
+    product's own monomial tables (through its Chebyshev expansion) agree:
+    a coefficient or exponent slip in any one of them shows as O(1) error."""
+    cheb, abs_sum, dom = M.model_cheb()
+    lo, hi = dom
+    rng = np.random.default_rng(13)
+    a = rng.dirichlet(np.ones(136), size=2).astype(np.float32)
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    for _ in range(200):
+        x, y = sorted(rng.uniform(lo, hi, 2))
+        co = np.zeros(4)
+        O.lib().oracle_model_coefs(float(x), float(y), co)
+        tx, ty = (2 * x - (lo + hi)) / (hi - lo), (2 * y - (lo + hi)) / (hi - lo)
+        Tx = np.polynomial.chebyshev.chebvander(tx, 7)
+        Ty = np.polynomial.chebyshev.chebvander(ty, 6)
+        for w in range(4):
+            v = (Tx @ cheb[w].astype(np.float64) @ Ty.T).item()
+            assert abs(v - co[w]) < 1e-4 * max(1.0, abs(co[w])), (w, x, y, v, co[w])
+        # device preProb vs host distance: prob = 1/(1+exp(pre)) with the floors
+        pre = O.preprob(a[0], a[1], float(x), float(y))
+        prob = O.tnf_dist_host(a[0], a[1], float(x), float(y))
+        want = 1.0 / (1 + math.exp(pre)) if pre < math.log(9.0) else 0.1
+        assert abs(prob - want) < 1e-12 or (pre >= math.log(9) - 1e-12 and abs(prob - 0.1) < 1e-12)
+
+
+def test_model_closed_forms_and_symmetry():
+    rng = np.random.default_rng(17)
+    v = rng.dirichlet(np.ones(136), size=3).astype(np.float32)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    ln9 = math.log(9.0)
+    for x in np.linspace(math.log10(1500), math.log10(500000), 9):
+        for y in np.linspace(math.log10(1500), math.log10(500000), 9):
+            co = np.zeros(4)
+            O.lib().oracle_model_coefs(float(x), float(y), co)
+            pre0 = O.preprob(v[0], v[0], float(x), float(y))  # d = 0
+            assert pre0 == -co[0] if -co[0] > ln9 else pre0 <= ln9
+            p_ab = O.preprob(v[0], v[1], float(x), float(y))
+            assert p_ab == O.preprob(v[1], v[0], float(y), float(x))  # symmetric, bitwise
+            assert co[1] > 19 and co[3] > 15  # both models decrease in d on the domain
+            ds = np.linspace(0, 1.4, 50)
+            pres = [O.lib().oracle_preprob_from_d(float(d), float(x), float(y)) for d in ds]
+            assert all(p2 <= p1 for p1, p2 in zip(pres, pres[1:]))
+            assert pres[-1] <= ln9  # far pairs fall to the clamped second model
+
+
+def test_cutoff_threshold_matches_reference_expression():
+    for c in (0.7, 0.9, 0.95, 0.999):
+        assert O.cutoff_preprob(c) == math.log((1.0 / (1.0 - c)) - 1.0)
+
+
+# ------------------------------------------------------------------- edges --
+def _small_tnf(n=240, genomes=3, seed=21, lmin=1500, lmax=60000):
+    cfg = M.gen_config(seed=seed, n_genomes=genomes, len_median=5000, len_sigma=1.0, len_min=lmin, len_max=lmax)
+    lens, _ = M.gen_lengths(cfg, n)
+    seqs = [M.gen_ascii(cfg, i, int(lens[i])) for i in range(n)]
+    c = np.stack([O.count_seq(s) for s in seqs])
+    return O.normalize_f32(c), O.logsize(lens)
+
+
+@pytest.mark.parametrize("K,cutoff", [(0, 0.95), (7, 0.95), (3, 0.8), (50, 0.999)])
+def test_oracle_edges_bruteforce(K, cutoff):
+    tnf, ls = _small_tnf()
+    n = tnf.shape[0]
+    T = math.log(1 / (1 - cutoff) - 1)
+    want = set()
+    for i in range(n):
+        cand = []
+        for j in range(n):
+            if j == i:
+                continue
+            p = O.preprob(tnf[i], tnf[j], ls[i], ls[j])
+            if p > T:
+                cand.append((1 - 1 / (1 + math.exp(p)), j))
+        cand.sort(key=lambda t: (-t[0], t[1]))
+        keep = cand if K == 0 else cand[:K]
+        want |= {(i, j) for _, j in keep if i < j}
+    f, t, s = O.edges(tnf, ls, cutoff, K)
+    got = set(zip(f.tolist(), t.tolist()))
+    assert got == want
+    assert np.all(f < t) and np.all(s > cutoff)
+    if K:
+        assert np.bincount(f, minlength=n).max() <= K
+
+
+def test_oracle_rowmax_bruteforce():
+    tnf, ls = _small_tnf(n=80)
+    n = tnf.shape[0]
+    idx = np.random.default_rng(2).permutation(n).astype(np.uint64)
+    got = O.rowmax(tnf, ls, idx, 20)
+    for p in range(20):
+        m = max([O.preprob(tnf[idx[p]], tnf[idx[q]], ls[idx[p]], ls[idx[q]]) for q in range(n) if q != p]
+                + [2.2250738585072014e-308])
+        assert got[p] == 1 - 1 / (1 + math.exp(m))
