@@ -89,7 +89,10 @@ int mdet_pack_ascii(const char *seq, const uint64_t *seq_off, uint64_t n, uint32
                     uint64_t packed_words, uint64_t *runs, uint64_t runs_cap, uint64_t *n_runs);
 
 /* log10(min(length, 500000)) in FP64, the per-contig size feature of the
- * pair model (cuda/Cuda2.cu:1535 logSizes). */
+ * pair model (cuda/Cuda2.cu:1535 logSizes).  Computed with the host libm, as
+ * the reference does while parsing: the device log10 differs from it in the
+ * last bit for ~1.7% of lengths in [1500, 500000], and the FP64 tie-breaker
+ * of the pair stage must see the same values as the reference. */
 int mdet_logsize_host(const uint64_t *lengths, uint64_t n, double *out);
 
 /* ------------------------------------------------------------ generator -- */
@@ -136,9 +139,6 @@ int mdet_tnf_count(mdet_ctx *ctx, const uint32_t *packed_dev, uint64_t n_bases, 
  * case-insensitive ACGT, anything else voids the windows that contain it. */
 int mdet_tnf_count_ascii(mdet_ctx *ctx, const char *seqs_dev, const uint64_t *index_dev, uint32_t n,
                          uint32_t *counts_dev, float *tnf_dev);
-
-/* log10(min(len, 500000)) on the device, for lengths_dev[n] -> out_dev[n]. */
-int mdet_logsize(mdet_ctx *ctx, const uint64_t *lengths_dev, uint32_t n, double *out_dev);
 
 /* ---------------------------------------------------------------- pairs -- */
 /* Precision of the all-pairs distance stage; every mode yields the same edge

@@ -74,7 +74,6 @@ _SIGS = {
     "mdet_tnf_count": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
                                  C.c_uint32, C.c_void_p, C.c_void_p]),
     "mdet_tnf_count_ascii": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
-    "mdet_logsize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "mdet_tnf_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double, C.c_uint32,
                                  C.c_int, C.POINTER(Edges), C.POINTER(PairStats)]),
     "mdet_tnf_preprob": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
@@ -282,9 +281,6 @@ class Ctx:
         _check(lib().mdet_tnf_count_ascii(self.h, C.c_void_p(seqs.ptr), C.c_void_p(index.ptr), n,
                                           C.c_void_p(counts.ptr if counts is not None else None),
                                           C.c_void_p(tnf.ptr)), "mdet_tnf_count_ascii")
-
-    def logsize(self, lengths: DevArray, n: int, out: DevArray):
-        _check(lib().mdet_logsize(self.h, C.c_void_p(lengths.ptr), n, C.c_void_p(out.ptr)), "mdet_logsize")
 
     def tnf_edges(self, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int,
                   precision: int = EXACT_F32, fetch: bool = True):

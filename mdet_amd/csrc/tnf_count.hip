@@ -256,13 +256,6 @@ __global__ __launch_bounds__(ASC_THREADS) void k_count_ascii(const char* __restr
     if (tid < 136 && fold[tid]) atomicAdd(&counts[(uint64_t)c * 136 + tid], fold[tid]);
 }
 
-__global__ void k_logsize(const uint64_t* __restrict__ len, uint32_t n, double* __restrict__ out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint64_t v = len[i] < 500000u ? len[i] : 500000u;
-    out[i] = log10((double)v);
-}
-
 }  // namespace
 
 using namespace mdet;
@@ -331,15 +324,6 @@ int mdet_tnf_count_ascii(mdet_ctx* ctx, const char* seqs_dev, const uint64_t* in
                            index_dev, n, choff, counts);
     MDET_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_normalize, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, counts, n, tnf_dev);
-    MDET_HIP(hipGetLastError());
-    return MDET_OK;
-}
-
-int mdet_logsize(mdet_ctx* ctx, const uint64_t* lengths_dev, uint32_t n, double* out_dev) {
-    MDET_TRY(check_ctx(ctx));
-    if (n == 0) return MDET_OK;
-    if (!lengths_dev || !out_dev) MDET_FAIL(MDET_E_INVALID, "mdet_logsize: null argument");
-    hipLaunchKernelGGL(k_logsize, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, lengths_dev, n, out_dev);
     MDET_HIP(hipGetLastError());
     return MDET_OK;
 }

@@ -1,0 +1,94 @@
+"""GPU parity of the counting path against the oracle and the reference's
+golden vectors: integer counts bit-exact, FP32 rows bit-exact."""
+import math
+
+import numpy as np
+import pytest
+
+import mdet_amd as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_count_packed(ctx, seqs):
+    packed, runs, off = M.pack_ascii(seqs)
+    n = len(seqs)
+    dp, dr, do = ctx.to_device(packed), ctx.to_device(runs if runs.size else np.zeros(2, np.uint64)), ctx.to_device(off)
+    cnt, tnf = ctx.empty((n, 136), np.uint32), ctx.empty((n, 136), np.float32)
+    ctx.tnf_count(dp, int(off[-1]), dr, len(runs) // 2, do, n, cnt, tnf)
+    out = cnt.numpy(), tnf.numpy()
+    for a in (dp, dr, do, cnt, tnf):
+        a.free()
+    return out
+
+
+def _gpu_count_ascii(ctx, seqs, gap=b"\n>x\n"):
+    # contigs laid out with junk between them, addressed by [begin | end]
+    buf, beg, end, pos = bytearray(), [], [], 0
+    for s in seqs:
+        buf += gap
+        beg.append(len(buf))
+        buf += s
+        end.append(len(buf))
+    n = len(seqs)
+    idx = np.array(beg + end, np.uint64)
+    ds, di = ctx.to_device(np.frombuffer(bytes(buf) or b"\0", np.uint8)), ctx.to_device(idx)
+    cnt, tnf = ctx.empty((n, 136), np.uint32), ctx.empty((n, 136), np.float32)
+    ctx.tnf_count_ascii(ds, di, n, cnt, tnf)
+    out = cnt.numpy(), tnf.numpy()
+    for a in (ds, di, cnt, tnf):
+        a.free()
+    return out
+
+
+def _same_f32(a, b):
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("path", ["packed", "ascii"])
+def test_count_matches_reference_golden(ctx, golden, path):
+    recs, meta, ref = golden["records"], golden["meta"], golden["tnf"]
+    seqs = [recs[k][1] for k in meta["rows"]]
+    print(f"golden fixture: {len(seqs)} contigs via {path}", flush=True)
+    cnt, tnf = (_gpu_count_packed if path == "packed" else _gpu_count_ascii)(ctx, seqs)
+    want = np.stack([O.count_seq(s) for s in seqs])
+    assert np.array_equal(cnt, want)
+    big = np.array([len(s) >= 2500 for s in seqs])
+    assert _same_f32(tnf[big], ref[big].astype(np.float32))
+
+
+def test_count_generated_2k(ctx):
+    cfg = M.gen_config(seed=3, n_genomes=20, len_median=6000, len_min=2500, len_max=100000, n_frac=0.005)
+    n = 2000
+    packed, runs, off, lens, gid = M.gen_packed(cfg, n)
+    print(f"2k contigs, {int(off[-1])} bases, {len(runs)//2} N runs", flush=True)
+    dp, dr, do = ctx.to_device(packed), ctx.to_device(runs), ctx.to_device(off)
+    cnt, tnf = ctx.empty((n, 136), np.uint32), ctx.empty((n, 136), np.float32)
+    ctx.tnf_count(dp, int(off[-1]), dr, len(runs) // 2, do, n, cnt, tnf)
+    c, t = cnt.numpy(), tnf.numpy()
+    seqs = [M.gen_ascii(cfg, i, int(lens[i])) for i in range(n)]
+    want = np.stack([O.count_seq(s) for s in seqs])
+    assert np.array_equal(c, want)
+    assert _same_f32(t, O.normalize_f32(want))
+    for a in (dp, dr, do, cnt, tnf):
+        a.free()
+
+
+def test_count_edge_cases(ctx):
+    rng = np.random.default_rng(9)
+    seqs = [b"", b"A", b"AC", b"ACG", b"ACGT", b"ACGTA", b"NNNNNNNN", b"ACGNTACG", b"acgtACGTnacgt"]
+    seqs += [bytes(rng.choice(list(b"ACGT"), size=int(k))) for k in rng.integers(5, 40, size=1500)]  # many tiny
+    long = bytearray(rng.choice(list(b"ACGT"), size=70000))
+    long[16383:16390] = b"NNNNNNN"  # N run across a chunk boundary
+    seqs.append(bytes(long))
+    seqs.append(b"N" * 20 + bytes(rng.choice(list(b"ACGT"), size=100)) + b"N" * 17)
+    print(f"edge cases: {len(seqs)} contigs", flush=True)
+    for path in (_gpu_count_packed, _gpu_count_ascii):
+        cnt, tnf = path(ctx, seqs)
+        want = np.stack([O.count_seq(s) for s in seqs])
+        assert np.array_equal(cnt, want)
+        ref = O.normalize_f32(want)
+        nan = np.isnan(ref)
+        assert np.array_equal(np.isnan(tnf), nan)  # no valid window: 0/0
+        assert _same_f32(tnf[~nan], ref[~nan])
