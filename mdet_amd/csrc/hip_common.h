@@ -40,6 +40,13 @@ struct mdet_ctx {
     size_t host_stage_bytes = 0;
     bool model_ready = false;  // FP32 model constants uploaded
     int num_cus = 0;
+    // cached prefilter radius for (min log size, max log size, threshold)
+    double radius_key[3] = {0, 0, 0};
+    double radius = 0;
+    bool radius_valid = false;
+    // cached MFMA work items for a padded row count
+    uint32_t items_npad = 0;
+    uint32_t nitems = 0;
 };
 
 namespace mdet {

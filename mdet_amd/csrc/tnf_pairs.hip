@@ -183,9 +183,27 @@ __device__ double model64(const float* __restrict__ a, const float* __restrict__
 // ------------------------------------------------------- wave utilities --
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
-// Reserve cnt slots per lane in a global list (all 64 lanes must call).
-// Returns this lane's first slot.
-__device__ inline uint64_t wave_reserve(uint32_t cnt, unsigned long long* counter) {
+// Survivor output without a hot counter: every wave appends to one of
+// NSHARD sub-buffers (wave id modulo NSHARD), each with its own counter on
+// its own 128-byte line, so the appends of ~10^6 waves never serialise on
+// one address.  A shard's counter keeps counting past its capacity so the
+// host learns the size needed; k_shard_scan/k_compact then pack the shards
+// into one dense list.
+constexpr int NSHARD = 2048;
+constexpr int SHARD_STRIDE = 16;  // counters 16 x 8 B = 128 B apart
+
+struct SurvOut {
+    uint2* buf;                // NSHARD * cap entries
+    unsigned long long* cnt;   // NSHARD * SHARD_STRIDE
+    unsigned long long cap;    // entries per shard
+};
+
+__device__ inline uint32_t wave_shard() {
+    return ((blockIdx.x * (blockDim.x >> 6)) + (threadIdx.x >> 6)) & (NSHARD - 1);
+}
+
+// All 64 lanes call; lane l gets the shard-local slot of its first item.
+__device__ inline unsigned long long wave_reserve_shard(uint32_t cnt, const SurvOut& o, uint32_t shard) {
     const uint32_t lane = lane_id();
     uint32_t x = cnt;
 #pragma unroll
@@ -195,9 +213,50 @@ __device__ inline uint64_t wave_reserve(uint32_t cnt, unsigned long long* counte
     }
     const uint32_t total = __shfl(x, 63, 64);
     unsigned long long base = 0;
-    if (lane == 63 && total) base = atomicAdd(counter, (unsigned long long)total);
+    if (lane == 63 && total) base = atomicAdd(&o.cnt[(size_t)shard * SHARD_STRIDE], (unsigned long long)total);
     base = __shfl(base, 63, 64);
     return base + (x - cnt);
+}
+
+__device__ inline void shard_put(const SurvOut& o, uint32_t shard, unsigned long long slot, uint32_t i, uint32_t j) {
+    if (slot < o.cap) o.buf[(size_t)shard * o.cap + slot] = make_uint2(i, j);
+}
+
+// offsets[s] = sum of min(cnt, cap) of shards < s; offsets[NSHARD] = total;
+// stat[0] = total, stat[1] = largest raw count (overflow check).
+__global__ __launch_bounds__(1024) void k_shard_scan(const unsigned long long* __restrict__ cnt,
+                                                     unsigned long long cap, unsigned long long* __restrict__ offsets,
+                                                     unsigned long long* __restrict__ stat) {
+    __shared__ unsigned long long sv[NSHARD];
+    __shared__ unsigned long long smax[1024];
+    const int t = threadIdx.x;
+    unsigned long long mx = 0;
+    for (int s = t; s < NSHARD; s += 1024) {
+        unsigned long long c = cnt[(size_t)s * SHARD_STRIDE];
+        mx = c > mx ? c : mx;
+        sv[s] = c < cap ? c : cap;
+    }
+    smax[t] = mx;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long acc = 0, m = 0;
+        for (int s = 0; s < NSHARD; ++s) {
+            offsets[s] = acc;
+            acc += sv[s];
+        }
+        offsets[NSHARD] = acc;
+        for (int k = 0; k < 1024; ++k) m = smax[k] > m ? smax[k] : m;
+        stat[0] = acc;
+        stat[1] = m;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact(const uint2* __restrict__ sbuf, unsigned long long cap,
+                                                 const unsigned long long* __restrict__ offsets,
+                                                 uint2* __restrict__ dense) {
+    const uint32_t s = blockIdx.x;
+    const unsigned long long b = offsets[s], c = offsets[s + 1] - b;
+    for (unsigned long long k = threadIdx.x; k < c; k += blockDim.x) dense[b + k] = sbuf[(size_t)s * cap + k];
 }
 
 // --------------------------------------------------------------- prep ----
@@ -262,42 +321,84 @@ __device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t
 // ks and lane l (row t*32 + l%32, k = 16*ks + 8*(l/32) .. +8), the 16 bytes
 // sit at element offset ((t*9 + ks)*64 + l)*8: one coalesced 1 KiB load per
 // wave serves an A or B operand of v_mfma_f32_32x32x16.
+//
+// K slots 136..143 carry the threshold so that the MFMA itself returns
+//   acc = a.b - h_i - h_j,   h = |a~|^2/2 - R2/4,
+// and "d~^2 < R2" is "acc > 0".  h is split into three pieces of the low
+// format (hh + hm + hl = h to ~2^-24 relative), multiplied by exact ones:
+//   column (B) role: slots 136..138 = 1, slots 139..141 = -hh, -hm, -hl
+//   row (A) role:    slots 136..138 = -hh, -hm, -hl, slots 139..141 = 1
+// The B-role rows are the main array; the A-role variant of the last k-step
+// half (k = 136..143) is kept separately (one uint4 per row).
 template <bool BF16>
-__global__ __launch_bounds__(256) void k_prep_lowp(const float* __restrict__ tnf, uint32_t n, uint32_t n_pad,
-                                                   uint4* __restrict__ frag, float* __restrict__ norm) {
+__device__ inline uint16_t to_low(float v) {
+    if (BF16) return to_bf16_rne(v);
+    _Float16 x = (_Float16)v;
+    return __builtin_bit_cast(uint16_t, x);
+}
+template <bool BF16>
+__device__ inline float from_low(uint16_t h) {
+    if (BF16) return from_bf16(h);
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+__device__ inline uint4 pack8(const uint16_t* h) {
+    uint4 w;
+    w.x = h[0] | ((uint32_t)h[1] << 16);
+    w.y = h[2] | ((uint32_t)h[3] << 16);
+    w.z = h[4] | ((uint32_t)h[5] << 16);
+    w.w = h[6] | ((uint32_t)h[7] << 16);
+    return w;
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_prep_lowp(const float* __restrict__ tnf, uint32_t n, uint32_t n_pad, float R2,
+                                                   uint4* __restrict__ frag, uint4* __restrict__ frag_a8) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= n_pad) return;
+    const uint32_t t = r >> 5;
     float part = 0.f;
-    if (lane < 18) {
+    uint16_t h[8];
+    if (lane < 17) {
         const int k0 = 8 * lane;
-        uint16_t h[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int k = k0 + e;
-            float v = (r < n && k < NT) ? tnf[(size_t)r * NT + k] : 0.f;
-            float back;
-            if (BF16) {
-                h[e] = to_bf16_rne(v);
-                back = from_bf16(h[e]);
-            } else {
-                _Float16 x = (_Float16)v;
-                h[e] = __builtin_bit_cast(uint16_t, x);
-                back = (float)x;
-            }
+            const float v = (r < n) ? tnf[(size_t)r * NT + k0 + e] : 0.f;
+            h[e] = to_low<BF16>(v);
+            const float back = from_low<BF16>(h[e]);
             part = fmaf(back, back, part);
         }
-        uint4 w;
-        w.x = h[0] | ((uint32_t)h[1] << 16);
-        w.y = h[2] | ((uint32_t)h[3] << 16);
-        w.z = h[4] | ((uint32_t)h[5] << 16);
-        w.w = h[6] | ((uint32_t)h[7] << 16);
-        const uint32_t t = r >> 5, ks = lane >> 1, half = lane & 1;
-        frag[((size_t)t * KS + ks) * 64 + (r & 31) + 32 * half] = w;
+        const uint32_t ks = lane >> 1, half = lane & 1;
+        frag[((size_t)t * KS + ks) * 64 + (r & 31) + 32 * half] = pack8(h);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
-    if (lane == 0) norm[r] = (r < n) ? part : __builtin_inff();
+    if (lane == 0) {
+        // padding rows get a large finite h (the split of +inf would be NaN)
+        const float big = BF16 ? 1e30f : 6.0e4f;
+        float hv = (r < n) ? fmaf(0.5f, part, -0.25f * R2) : big;
+        uint16_t p[3];
+        float rem = hv;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            p[q] = to_low<BF16>(rem);
+            rem = rem - from_low<BF16>(p[q]);
+        }
+        const uint16_t one = to_low<BF16>(1.f);
+        uint16_t nb[8], na[8];
+        for (int q = 0; q < 3; ++q) {
+            nb[q] = one;
+            nb[3 + q] = p[q] ^ 0x8000u;  // negate
+            na[q] = p[q] ^ 0x8000u;
+            na[3 + q] = one;
+        }
+        nb[6] = nb[7] = na[6] = na[7] = 0;
+        frag[((size_t)t * KS + 8) * 64 + (r & 31) + 32] = pack8(nb);
+        frag_a8[(size_t)t * 64 + (r & 31) + 32] = pack8(na);
+    } else if (lane == 16) {
+        // the first half of k-step 8 (k = 128..135) is plain TNF in both roles
+        frag_a8[(size_t)t * 64 + (r & 31)] = pack8(h);
+    }
 }
 
 // ------------------------------------------------------ FP32 prefilter ----
@@ -307,8 +408,7 @@ __global__ __launch_bounds__(256) void k_prep_lowp(const float* __restrict__ tnf
 constexpr int F_T = 128, F_KC = 8;
 
 __global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__ tnft, uint32_t n, uint32_t n_pad,
-                                                       uint32_t ntiles, float R2, uint2* __restrict__ surv,
-                                                       unsigned long long cap, unsigned long long* __restrict__ counter) {
+                                                       uint32_t ntiles, float R2, SurvOut out) {
     __shared__ float As[F_KC][F_T], Bs[F_KC][F_T];
     // linear index -> (bi, bj), bi <= bj, rows of the triangle of length ntiles - bi
     const uint64_t lin = blockIdx.x;
@@ -365,179 +465,214 @@ __global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__
             cnt += (acc[a][b] < R2 && i < j && j < n) ? 1u : 0u;
         }
     if (__any(cnt != 0)) {
-        uint64_t pos = wave_reserve(cnt, counter);
+        const uint32_t shard = wave_shard();
+        unsigned long long pos = wave_reserve_shard(cnt, out, shard);
 #pragma unroll
         for (int a = 0; a < 8; ++a)
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
                 const uint32_t i = rb + ty + 16 * a, j = cb + tx + 16 * b;
-                if (acc[a][b] < R2 && i < j && j < n) {
-                    if (pos < cap) surv[pos] = make_uint2(i, j);
-                    ++pos;
-                }
+                if (acc[a][b] < R2 && i < j && j < n) shard_put(out, shard, pos++, i, j);
             }
     }
 }
 
 // ------------------------------------------------------- MFMA prefilter ---
-// Block = 4 waves = 256 rows (64 per wave, two 32-row MFMA tiles held in
+// Block = 4 waves = 256 rows (64 per wave: two 32-row MFMA tiles held in
 // registers for the whole column sweep); columns stream through LDS in
-// steps of 128 (four 32-column tiles, 36 KiB in fragment order, loaded once
-// per block and read by all 4 waves).  Per step a wave issues 2 x 4 x 9
-// v_mfma_f32_32x32x16 into 8 independent accumulators, then tests
-//   d~^2 = n_i + n_j - 2 g < R2   <=>   g - n_j/2 > n_i/2 - R2/2
-// which is one subtract and one compare per pair.
+// steps of 128 (four 32-column tiles, 36 KiB in fragment order), double
+// buffered and filled by global_load_lds_dwordx4 (no register staging).  Per
+// step a wave issues 2 x 4 x 9 v_mfma_f32_32x32x16, as two 64-column halves
+// of 4 independent accumulator chains each; with the threshold folded into K, the test of a pair is
+// "acc > 0": one compare, counted per 32x32 tile with a wave ballot.  Only
+// steps with survivors enter the extraction pass, whose cost scales with the
+// number of (tile, accumulator) slots that hold survivors.
 constexpr int M_ROWS = 256, M_STEP = 128, M_SEG = 2048;
 constexpr int M_STEP_U4 = M_STEP / 32 * KS * 64;  // uint4 per B step (2304)
 
 template <bool BF16>
-__global__ __launch_bounds__(256) void k_prefilter_mfma(const uint4* __restrict__ frag, const float* __restrict__ norm,
-                                                        const uint4* __restrict__ items, uint32_t n, float R2,
-                                                        uint2* __restrict__ surv, unsigned long long cap,
-                                                        unsigned long long* __restrict__ counter) {
-    __shared__ uint4 Bl[M_STEP_U4];
+__device__ inline f32x16_t mfma32(uint4 a, uint4 b, f32x16_t c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                       c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b), c, 0,
+                                                  0, 0);
+}
+
+__device__ inline void issue_btile(const uint4* __restrict__ frag, uint32_t c0, uint4* lds, int tid, int wave) {
+    const uint4* src = frag + (size_t)(c0 / 32) * KS * 64;
+#pragma unroll
+    for (int m = 0; m < KS; ++m)
+        __builtin_amdgcn_global_load_lds((const void*)(src + 256 * m + tid),
+                                         (__attribute__((address_space(3))) void*)(lds + 256 * m + 64 * wave), 16, 0,
+                                         0);
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(256, 2) void k_prefilter_mfma(const uint4* __restrict__ frag,
+                                                           const uint4* __restrict__ frag_a8,
+                                                           const uint4* __restrict__ items, uint32_t n,
+                                                           SurvOut out) {
+    __shared__ uint4 Bl[2][M_STEP_U4];
     const uint4 it = items[blockIdx.x];
     const uint32_t rb = it.x, c_begin = it.y, c_end = it.z;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t row0 = rb * M_ROWS + wave * 64;  // this wave's 64 rows
     const uint32_t t0 = row0 >> 5;
 
+    issue_btile(frag, c_begin, Bl[0], tid, wave);
     uint4 a[2][KS];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int rt = 0; rt < 2; ++rt) {
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) a[rt][ks] = frag[((size_t)(t0 + rt) * KS + ks) * 64 + lane];
-    // row thresholds: accumulator r of row tile rt holds row
-    // row0 + 32*rt + 8*(r/4) + 4*(lane/32) + r%4, column lane%32
-    float ei[2][16];
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const uint32_t i = row0 + 32 * rt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            ei[rt][r] = fmaf(0.5f, norm[i], -0.5f * R2);
-        }
+        for (int ks = 0; ks < KS - 1; ++ks) a[rt][ks] = frag[((size_t)(t0 + rt) * KS + ks) * 64 + lane];
+        a[rt][KS - 1] = frag_a8[(size_t)(t0 + rt) * 64 + lane];
+    }
+    // accumulator r of row tile rt holds row row0 + 32*rt + rbase + r%4 + 8*(r/4)
+    const uint32_t rbase = 4 * (lane >> 5);
+    const uint32_t shard = wave_shard();
 
-    uint4 pf[KS];
-    const uint4* src = frag + (size_t)(c_begin / 32) * KS * 64;
-#pragma unroll
-    for (int m = 0; m < KS; ++m) pf[m] = src[tid + 256 * m];
-
-    for (uint32_t c0 = c_begin; c0 < c_end; c0 += M_STEP) {
+    int buf = 0;
+    for (uint32_t c0 = c_begin; c0 < c_end; c0 += M_STEP, buf ^= 1) {
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+        if (c0 + M_STEP < c_end) issue_btile(frag, c0 + M_STEP, Bl[buf ^ 1], tid, wave);
+        const uint4* B = Bl[buf];
+        // pairs with j <= i exist only where the step meets the diagonal
+        const bool diag = c0 < row0 + 64;
 #pragma unroll
-        for (int m = 0; m < KS; ++m) Bl[tid + 256 * m] = pf[m];
-        __syncthreads();
-        if (c0 + M_STEP < c_end) {
-            const uint4* nsrc = frag + (size_t)((c0 + M_STEP) / 32) * KS * 64;
-#pragma unroll
-            for (int m = 0; m < KS; ++m) pf[m] = nsrc[tid + 256 * m];
-        }
-        f32x16_t acc[2][4];
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f32x16_t{};
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            uint4 b[4];
-#pragma unroll
-            for (int ct = 0; ct < 4; ++ct) b[ct] = Bl[(ct * KS + ks) * 64 + lane];
+        for (int hc = 0; hc < 2; ++hc) {  // two 64-column halves of the step
+            f32x16_t acc[2][2];
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct) {
-                    if (BF16)
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            __builtin_bit_cast(bf16x8_t, a[rt][ks]), __builtin_bit_cast(bf16x8_t, b[ct]), acc[rt][ct],
-                            0, 0, 0);
-                    else
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
-                            __builtin_bit_cast(f16x8_t, a[rt][ks]), __builtin_bit_cast(f16x8_t, b[ct]), acc[rt][ct],
-                            0, 0, 0);
-                }
-        }
-        float fj[4];
+                for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = f32x16_t{};
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) fj[ct] = 0.5f * norm[c0 + 32 * ct + (lane & 31)];
-        bool hit = false;
+            for (int ks = 0; ks < KS; ++ks) {
+                uint4 b[2];
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+                for (int ct = 0; ct < 2; ++ct) b[ct] = B[((2 * hc + ct) * KS + ks) * 64 + lane];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
+                for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) hit |= (acc[rt][ct][r] - fj[ct]) > ei[rt][r];
-        if (__any(hit)) {
-            uint32_t cnt = 0;
+                    for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma32<BF16>(a[rt][ks], b[ct], acc[rt][ct]);
+            }
+            const uint32_t jl = c0 + 64 * hc + (lane & 31);
+            uint32_t tcnt[2][2];
+            uint32_t total = 0;
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
+                for (int ct = 0; ct < 2; ++ct) {
+                    uint32_t c = 0;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const uint32_t i = row0 + 32 * rt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                        const uint32_t j = c0 + 32 * ct + (lane & 31);
-                        cnt += ((acc[rt][ct][r] - fj[ct]) > ei[rt][r] && i < j) ? 1u : 0u;
+                        const uint32_t i = row0 + 32 * rt + rbase + (r & 3) + 8 * (r >> 2);
+                        const bool h = acc[rt][ct][r] > 0.f && (!diag || i < jl + 32 * ct);
+                        c += (uint32_t)__popcll(__ballot(h));
                     }
-            uint64_t pos = wave_reserve(cnt, counter);
+                    tcnt[rt][ct] = c;
+                    total += c;
+                }
+            if (total) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&out.cnt[(size_t)shard * SHARD_STRIDE], (unsigned long long)total);
+                base = __shfl(base, 0, 64);
+                const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
+                for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
+                    for (int ct = 0; ct < 2; ++ct) {
+                        if (tcnt[rt][ct] == 0) continue;
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-                        const uint32_t i = row0 + 32 * rt + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                        const uint32_t j = c0 + 32 * ct + (lane & 31);
-                        if ((acc[rt][ct][r] - fj[ct]) > ei[rt][r] && i < j) {
-                            if (pos < cap) surv[pos] = make_uint2(i, j);
-                            ++pos;
+                        for (int r = 0; r < 16; ++r) {
+                            const uint32_t i = row0 + 32 * rt + rbase + (r & 3) + 8 * (r >> 2);
+                            const uint32_t j = jl + 32 * ct;
+                            const bool h = acc[rt][ct][r] > 0.f && (!diag || i < j);
+                            const uint64_t m = __ballot(h);
+                            if (m) {
+                                if (h) shard_put(out, shard, base + (uint64_t)__popcll(m & below), i, j);
+                                base += (unsigned long long)__popcll(m);
+                            }
                         }
                     }
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------- refine --
+// Four lanes per survivor: each lane streams 1/4 of the two FP32 rows
+// (float4 q = lane%4 + 4m), two shuffles complete d^2, the quad's first lane
+// evaluates the model.  16 survivors per wave in flight, 64-byte contiguous
+// segments per row per load instruction.
 __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, uint64_t ns,
                                                 const float* __restrict__ tnf, const double* __restrict__ ls,
                                                 const float* __restrict__ tvec, double T, Cand* __restrict__ cand,
-                                                unsigned long long cap, unsigned long long* __restrict__ counters) {
+                                                unsigned long long* __restrict__ counters) {
+    __shared__ unsigned int s_keep, s_adj;
+    if (threadIdx.x == 0) s_keep = s_adj = 0;
+    __syncthreads();
     const float T32 = (float)T;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x;
+    const uint32_t q = threadIdx.x & 3;
+    const uint64_t stride = (uint64_t)gridDim.x * (blockDim.x / 4);
+    const uint64_t first = (uint64_t)blockIdx.x * (blockDim.x / 4);
+    unsigned int nkeep = 0, nadj = 0;
     for (uint64_t base = first; base < ns; base += stride) {
-        const uint64_t idx = base + threadIdx.x;
-        bool keep = false, exact = false;
+        const uint64_t idx = base + (threadIdx.x >> 2);
         uint2 p = make_uint2(0, 0);
-        float key = 0.f;
+        float part = 0.f;
         if (idx < ns) {
             p = surv[idx];
-            const float* a = tnf + (size_t)p.x * NT;
-            const float* b = tnf + (size_t)p.y * NT;
-            const float d = sqrtf(dist2_f32(a, b));
+            const float4* a4 = reinterpret_cast<const float4*>(tnf + (size_t)p.x * NT);
+            const float4* b4 = reinterpret_cast<const float4*>(tnf + (size_t)p.y * NT);
+            float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+            for (int m = 0; m < 9; ++m) {
+                const uint32_t k = q + 4 * m;
+                if (k < NT / 4) {
+                    const float4 x = a4[k], y = b4[k];
+                    const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
+                    s0 = fmaf(d0, d0, s0);
+                    s1 = fmaf(d1, d1, s1);
+                    s0 = fmaf(d2, d2, s0);
+                    s1 = fmaf(d3, d3, s1);
+                }
+            }
+            part = s0 + s1;
+        }
+        part += __shfl_xor(part, 1, 64);
+        part += __shfl_xor(part, 2, 64);
+        if (idx < ns && q == 0) {
+            const float d = sqrtf(part);
             const Pre32 m = model32(d, tvec[p.x], tvec[p.y]);
             const bool amb = fabsf(m.pre - T32) <= EPS_PRE || fabsf(m.pre1 - FLOOR32) <= EPS_PRE;
+            bool keep;
+            float key;
             if (amb) {
-                const double p64 = model64(a, b, ls[p.x], ls[p.y]);
+                const double p64 = model64(tnf + (size_t)p.x * NT, tnf + (size_t)p.y * NT, ls[p.x], ls[p.y]);
                 keep = p64 > T;
                 key = (float)p64;
-                exact = true;
+                ++nadj;
             } else {
                 keep = m.pre > T32;
                 key = m.pre;
             }
-        }
-        const uint64_t nadj = __popcll(__ballot(exact));
-        if (lane_id() == 0 && nadj) atomicAdd(&counters[1], (unsigned long long)nadj);
-        uint64_t pos = wave_reserve(keep ? 1u : 0u, &counters[0]);
-        if (keep && pos < cap) {
+            nkeep += keep ? 1u : 0u;
             Cand c;
             c.i = p.x;
             c.j = p.y;
             c.pre = key;
-            c.flags = exact ? 1u : 0u;
-            cand[pos] = c;
+            c.flags = (amb ? 1u : 0u) | (keep ? 2u : 0u);
+            cand[idx] = c;
         }
+    }
+    if (nkeep) atomicAdd(&s_keep, nkeep);
+    if (nadj) atomicAdd(&s_adj, nadj);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_keep) atomicAdd(&counters[0], (unsigned long long)s_keep);
+        if (s_adj) atomicAdd(&counters[1], (unsigned long long)s_adj);
     }
 }
 
@@ -545,8 +680,10 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
 __global__ void k_degree(const Cand* __restrict__ cand, uint64_t nc, uint32_t* __restrict__ deg) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nc) return;
-    atomicAdd(&deg[cand[k].i], 1u);
-    atomicAdd(&deg[cand[k].j], 1u);
+    const Cand c = cand[k];
+    if (!(c.flags & 2u)) return;
+    atomicAdd(&deg[c.i], 1u);
+    atomicAdd(&deg[c.j], 1u);
 }
 
 __global__ void k_scatter(const Cand* __restrict__ cand, uint64_t nc, const uint64_t* __restrict__ rowptr,
@@ -554,6 +691,7 @@ __global__ void k_scatter(const Cand* __restrict__ cand, uint64_t nc, const uint
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nc) return;
     const Cand c = cand[k];
+    if (!(c.flags & 2u)) return;
     adj[rowptr[c.i] + atomicAdd(&cursor[c.i], 1u)] = Adj{c.j, c.pre};
     adj[rowptr[c.j] + atomicAdd(&cursor[c.j], 1u)] = Adj{c.i, c.pre};
 }
@@ -803,6 +941,14 @@ int exclusive_scan_u32_to_u64(mdet_ctx* ctx, const uint32_t* in, uint64_t* out, 
 
 // Prefilter items for the MFMA kernel: (row block, column begin, column end).
 int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nitems) {
+    if (ctx->items_npad == n_pad && ctx->nitems) {
+        auto it = ctx->bufs.find("pairs.items");
+        if (it != ctx->bufs.end()) {
+            *items_dev = static_cast<uint4*>(it->second.p);
+            *nitems = ctx->nitems;
+            return MDET_OK;
+        }
+    }
     std::vector<uint4> items;
     const uint32_t nrb = n_pad / M_ROWS;
     for (uint32_t rb = 0; rb < nrb; ++rb)
@@ -813,6 +959,8 @@ int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nite
     MDET_HIP(hipMemcpyAsync(*items_dev, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice,
                             ctx->stream));
     MDET_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->items_npad = n_pad;
+    ctx->nitems = *nitems;
     return MDET_OK;
 }
 
@@ -876,22 +1024,35 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
         memcpy(&xlo, &a, 8);
         memcpy(&xhi, &b, 8);
     }
-    const double R = model_dthr_max(xlo, xhi, T);
+    if (!(ctx->radius_valid && ctx->radius_key[0] == xlo && ctx->radius_key[1] == xhi && ctx->radius_key[2] == T)) {
+        ctx->radius = model_dthr_max(xlo, xhi, T);
+        ctx->radius_key[0] = xlo;
+        ctx->radius_key[1] = xhi;
+        ctx->radius_key[2] = T;
+        ctx->radius_valid = true;
+    }
+    const double R = ctx->radius;
     mdet_edges result{0, nullptr, nullptr, nullptr};
 
     if (R > 0) {
         // -- prefilter
         const uint32_t n_pad = (n + M_ROWS - 1) / M_ROWS * M_ROWS;
-        unsigned long long* counters;  // [0] survivors [1] candidates [2] adjudicated
-        MDET_TRY(ctx_buf(ctx, "pairs.counters", 8, &counters));
-        uint64_t cap = std::max<uint64_t>(1u << 22, (uint64_t)n * 512);
-        auto it_cap = ctx->bufs.find("pairs.surv");
-        if (it_cap != ctx->bufs.end()) cap = std::max<uint64_t>(cap, it_cap->second.bytes / sizeof(uint2));
-        uint2* surv = nullptr;
+        unsigned long long *scnt, *soff, *sstat, *rc;
+        MDET_TRY(ctx_buf(ctx, "pairs.shard_cnt", (size_t)NSHARD * SHARD_STRIDE, &scnt));
+        MDET_TRY(ctx_buf(ctx, "pairs.shard_off", (size_t)NSHARD + 1, &soff));
+        MDET_TRY(ctx_buf(ctx, "pairs.stat", 8, &sstat));
+        rc = sstat + 4;  // [4] kept candidates [5] FP64-adjudicated
+        uint64_t shard_cap = std::max<uint64_t>(2048, ((uint64_t)n * 256) / NSHARD);
+        auto it_cap = ctx->bufs.find("pairs.sbuf");
+        if (it_cap != ctx->bufs.end())
+            shard_cap = std::max<uint64_t>(shard_cap, it_cap->second.bytes / sizeof(uint2) / NSHARD);
+        uint2 *sbuf = nullptr, *surv = nullptr;
         uint64_t ns = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
-            MDET_TRY(ctx_buf(ctx, "pairs.surv", cap, &surv));
-            MDET_HIP(hipMemsetAsync(counters, 0, 8 * sizeof(unsigned long long), ctx->stream));
+            MDET_TRY(ctx_buf(ctx, "pairs.sbuf", (size_t)NSHARD * shard_cap, &sbuf));
+            MDET_HIP(hipMemsetAsync(scnt, 0, (size_t)NSHARD * SHARD_STRIDE * 8, ctx->stream));
+            MDET_HIP(hipMemsetAsync(sstat, 0, 8 * 8, ctx->stream));
+            const SurvOut so{sbuf, scnt, (unsigned long long)shard_cap};
             if (precision == MDET_EXACT_F32) {
                 float* tnft;
                 MDET_TRY(ctx_buf(ctx, "pairs.tnft", (size_t)NT * n_pad, &tnft));
@@ -902,57 +1063,67 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
                 if (ntri > 0x7fffffffull) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: too many tiles");
                 const float R2 = (float)(R * R * (1.0 + 1e-4)) + 1e-6f;
                 hipLaunchKernelGGL(k_prefilter_f32, dim3((unsigned)ntri), dim3(256), 0, ctx->stream, tnft, n, n_pad,
-                                   ntiles, R2, surv, (unsigned long long)cap, counters);
+                                   ntiles, R2, so);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
-                uint4* frag;
-                float* norm;
-                MDET_TRY(ctx_buf(ctx, "pairs.frag", (size_t)n_pad / 32 * KS * 64, &frag));
-                MDET_TRY(ctx_buf(ctx, "pairs.norm", n_pad, &norm));
-                if (bf)
-                    hipLaunchKernelGGL(k_prep_lowp<true>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
-                                       n_pad, frag, norm);
-                else
-                    hipLaunchKernelGGL(k_prep_lowp<false>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
-                                       n_pad, frag, norm);
-                uint4* items;
-                uint32_t nitems;
-                MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
-                if (attempt == 0) MDET_TRY(tm.record(1));
                 // |d~ - d| <= |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit
                 // roundoff of the format), plus FP32 accumulation slack
                 const double u = bf ? 1.0 / 256.0 : 1.0 / 2048.0;
                 const double Rm = R + 2.0 * u * 1.0001 + 2e-4;
                 const float R2 = (float)(Rm * Rm) + 1e-4f;
+                uint4 *frag, *frag_a8;
+                MDET_TRY(ctx_buf(ctx, "pairs.frag", (size_t)n_pad / 32 * KS * 64, &frag));
+                MDET_TRY(ctx_buf(ctx, "pairs.frag_a8", (size_t)n_pad / 32 * 64, &frag_a8));
                 if (bf)
-                    hipLaunchKernelGGL(k_prefilter_mfma<true>, dim3(nitems), dim3(256), 0, ctx->stream, frag, norm,
-                                       items, n, R2, surv, (unsigned long long)cap, counters);
+                    hipLaunchKernelGGL(k_prep_lowp<true>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
+                                       n_pad, R2, frag, frag_a8);
                 else
-                    hipLaunchKernelGGL(k_prefilter_mfma<false>, dim3(nitems), dim3(256), 0, ctx->stream, frag, norm,
-                                       items, n, R2, surv, (unsigned long long)cap, counters);
+                    hipLaunchKernelGGL(k_prep_lowp<false>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
+                                       n_pad, R2, frag, frag_a8);
+                uint4* items;
+                uint32_t nitems;
+                MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                if (attempt == 0) MDET_TRY(tm.record(1));
+                if (bf)
+                    hipLaunchKernelGGL(k_prefilter_mfma<true>, dim3(nitems), dim3(256), 0, ctx->stream, frag, frag_a8,
+                                       items, n, so);
+                else
+                    hipLaunchKernelGGL(k_prefilter_mfma<false>, dim3(nitems), dim3(256), 0, ctx->stream, frag,
+                                       frag_a8, items, n, so);
             }
             MDET_HIP(hipGetLastError());
             if (attempt == 0) MDET_TRY(tm.record(2));
-            unsigned long long c;
-            MDET_TRY(read_scalar(ctx, counters, &c));
-            ns = c;
-            if (ns <= cap) break;
+            hipLaunchKernelGGL(k_shard_scan, dim3(1), dim3(1024), 0, ctx->stream, scnt, (unsigned long long)shard_cap,
+                               soff, sstat);
+            unsigned long long st2[2];
+            {
+                void* st_host;
+                MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
+                MDET_HIP(hipMemcpyAsync(st_host, sstat, 16, hipMemcpyDeviceToHost, ctx->stream));
+                MDET_HIP(hipStreamSynchronize(ctx->stream));
+                memcpy(st2, st_host, 16);
+            }
+            if (st2[1] <= shard_cap) {
+                ns = st2[0];
+                break;
+            }
             if (attempt == 1) MDET_FAIL(MDET_E_STATE, "mdet_tnf_edges: survivor count grew between passes");
-            cap = ns + ns / 8;
+            shard_cap = st2[1] + st2[1] / 4 + 64;
         }
+        MDET_TRY(ctx_buf(ctx, "pairs.surv", std::max<uint64_t>(ns, 1), &surv));
+        hipLaunchKernelGGL(k_compact, dim3(NSHARD), dim3(256), 0, ctx->stream, sbuf, (unsigned long long)shard_cap,
+                           soff, surv);
+        MDET_HIP(hipGetLastError());
         st.survivors = ns;
         st.pairs_evaluated = st.pairs_total;
 
-        // -- refine
+        // -- refine (in place: cand[k] belongs to surv[k])
         Cand* cand;
         MDET_TRY(ctx_buf(ctx, "pairs.cand", std::max<uint64_t>(ns, 1), &cand));
-        unsigned long long* rc;
-        MDET_TRY(ctx_buf(ctx, "pairs.rcount", 4, &rc));
-        MDET_HIP(hipMemsetAsync(rc, 0, 4 * sizeof(unsigned long long), ctx->stream));
         if (ns) {
-            const uint64_t blocks = std::min<uint64_t>((ns + 255) / 256, (uint64_t)ctx->num_cus * 32);
+            const uint64_t blocks = std::min<uint64_t>((ns + 63) / 64, (uint64_t)ctx->num_cus * 32);
             hipLaunchKernelGGL(k_refine, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, surv, ns, tnf_dev,
-                               logsize_dev, tvec, T, cand, (unsigned long long)ns, rc);
+                               logsize_dev, tvec, T, cand, rc);
             MDET_HIP(hipGetLastError());
         }
         MDET_TRY(tm.record(3));
@@ -964,8 +1135,8 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
             MDET_HIP(hipStreamSynchronize(ctx->stream));
             memcpy(rch, st_host, sizeof rch);
         }
-        const uint64_t nc = rch[0];
-        st.candidates = nc;
+        const uint64_t nc = ns;  // candidate records (kept ones flagged)
+        st.candidates = rch[0];
         st.adjudicated = rch[1];
 
         // -- select
@@ -979,7 +1150,7 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
         MDET_TRY(ctx_buf(ctx, "sel.outcnt", (size_t)n + 1, &out_cnt));
         MDET_TRY(ctx_buf(ctx, "sel.rowptr", (size_t)n + 1, &rowptr));
         MDET_TRY(ctx_buf(ctx, "sel.eoff", (size_t)n + 1, &eoff));
-        MDET_TRY(ctx_buf(ctx, "sel.adj", std::max<uint64_t>(2 * nc, 1), &adj));
+        MDET_TRY(ctx_buf(ctx, "sel.adj", std::max<uint64_t>(2 * st.candidates, 1), &adj));
         MDET_TRY(ctx_buf(ctx, "sel.err", 4, &err));
         capped = reinterpret_cast<unsigned long long*>(err + 2);
         MDET_HIP(hipMemsetAsync(deg, 0, ((size_t)n + 1) * 4, ctx->stream));
