@@ -146,6 +146,8 @@ int mdet_tnf_count_ascii(mdet_ctx *ctx, const char *seqs_dev, const uint64_t *in
 #define MDET_EXACT_F32 0   /* FP32 sum of squared differences for every pair */
 #define MDET_BF16_REFINE 1 /* BF16 MFMA Gram prefilter + FP32 refinement */
 #define MDET_F16_REFINE 2  /* FP16 MFMA Gram prefilter + FP32 refinement */
+#define MDET_LEARNED 3     /* learned embedding proposals + FP32 refinement
+                            * (mdet_tnf_edges_learned; recall measured, not guaranteed) */
 
 typedef struct mdet_edges {
     uint64_t count;
@@ -184,6 +186,62 @@ typedef struct mdet_pair_stats {
  */
 int mdet_tnf_edges(mdet_ctx *ctx, const float *tnf_dev, const double *logsize_dev, uint32_t n, double cutoff,
                    uint32_t max_edges, int precision, mdet_edges *out, mdet_pair_stats *stats);
+
+/* The pairs the last mdet_tnf_edges / mdet_tnf_edges_learned call on ctx
+ * evaluated exactly: *count records of 16 bytes {uint32 i, uint32 j,
+ * float preProb, uint32 flags} (flags bit 0: decided by the FP64
+ * tie-breaker, bit 1: preProb above the threshold), device memory owned by
+ * ctx.  These label the training pairs of the learned filter. */
+int mdet_last_pairs(mdet_ctx *ctx, const void **records_dev, uint64_t *count);
+
+/* ---------------------------------------------------- learned filter -- */
+/* No reference counterpart: the reference evaluates every pair.  An MLP
+ * 136 -> 256 -> 256 -> 32 (GELU, BF16 weights, FP32 accumulation) maps a
+ * standardised TNF row to an embedding; trained with a contrastive margin
+ * loss on pairs the exact path labels, so that a small embedding distance
+ * proposes the pairs that can clear the edge threshold. */
+typedef struct mdet_mlp mdet_mlp;
+int mdet_mlp_create(mdet_ctx *ctx, uint64_t seed, mdet_mlp **out);
+int mdet_mlp_destroy(mdet_mlp *mlp);
+uint64_t mdet_mlp_param_count(void); /* 109,088 */
+/* FP32 master parameters, layout W1[256x136] b1[256] W2[256x256] b2[256]
+ * W3[32x256] b3[32] (row-major, out x in); host arrays. */
+int mdet_mlp_get_params(mdet_mlp *mlp, float *out, uint64_t count);
+int mdet_mlp_set_params(mdet_mlp *mlp, const float *in, uint64_t count);
+/* Input standardisation x' = (tnf - mean) * istd, per column (host arrays). */
+int mdet_mlp_set_input_norm(mdet_mlp *mlp, const float *mean136, const float *istd136);
+/* Loss L = mean y*D + (1-y)*relu(margin - D), D = |e_i - e_j|^2, over the
+ * batch of pairs (pi_dev[b], pj_dev[b], label_dev[b] in {0,1}); grad_out
+ * (host, param_count floats) receives dL/dparams at the BF16-rounded weights
+ * the forward pass uses; either output may be NULL. */
+int mdet_mlp_grad(mdet_mlp *mlp, const float *tnf_dev, const uint32_t *pi_dev, const uint32_t *pj_dev,
+                  const float *label_dev, uint32_t batch, float margin, float *grad_out, float *loss_out);
+/* One Adam step (beta 0.9 / 0.999) on that loss; loss_out may be NULL. */
+int mdet_mlp_train_step(mdet_mlp *mlp, const float *tnf_dev, const uint32_t *pi_dev, const uint32_t *pj_dev,
+                        const float *label_dev, uint32_t batch, float lr, float margin, float *loss_out);
+/* Embeddings emb_dev[n x 32] of TNF rows tnf_dev[n x 136]. */
+int mdet_mlp_embed(mdet_mlp *mlp, const float *tnf_dev, uint32_t n, float *emb_dev);
+/* Edge list as mdet_tnf_edges, but only pairs whose BF16 embedding distance^2
+ * is below radius2 are evaluated (MFMA Gram of the 32-dimensional
+ * embeddings); those are decided exactly as in the other modes.  Edges of
+ * pairs the network fails to propose are missed: recall is measured against
+ * mdet_tnf_edges. */
+int mdet_tnf_edges_learned(mdet_ctx *ctx, mdet_mlp *mlp, const float *tnf_dev, const double *logsize_dev, uint32_t n,
+                           double cutoff, uint32_t max_edges, float radius2, mdet_edges *out,
+                           mdet_pair_stats *stats);
+
+/*
+ * Sampled row maxima for the automatic cutoff (pTNF = 0): for each sampled
+ * position p < n_sample, max_out_dev[p] = 1 - 1/(1 + exp(m)) with m the
+ * largest preProb between contig idx_dev[p] and every contig idx_dev[q],
+ * q != p, q < n (and m >= DBL_MIN).  idx_dev is a permutation of 0..n-1 whose
+ * first n_sample entries are the sample.  Replaces get_tnf_max_prob_sample
+ * and launch_tnf_max_prob_sample_kernel_multi (cuda/Cuda2.cu:281-314,
+ * 356-379); the maximum is exact (FP32 scan, FP64 re-evaluation of every
+ * pair within the FP32 error band of the row maximum).
+ */
+int mdet_tnf_rowmax(mdet_ctx *ctx, const float *tnf_dev, const double *logsize_dev, uint32_t n,
+                    const uint64_t *idx_dev, uint32_t n_sample, double *max_out_dev);
 
 /* The FP32 form of the model the device evaluates: tensor Chebyshev
  * coefficients (4 x 8 x 7 floats: b1, c1, b2, c2; row p = degree in the

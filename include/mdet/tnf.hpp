@@ -1,0 +1,278 @@
+// C++ drop-in for MdeT's TNF contig-graph path, over the C ABI in tnf.h.
+//
+// Mirrors the call surface the reference's main() uses (cuda/Cuda2.cu):
+//   Graph                      cuda/Cuda2.cu:432-454 (from / to / sTNF)
+//   TnfPath::compute_tnf       TNF_CAL + device copies, cuda/Cuda2.cu:1823-1874
+//   TnfPath::gen_tnf_graph_sample   cuda/Cuda2.cu:972-1025 (rand()-driven)
+//   TnfPath::select_ptnf       the pTNF logic of main(), cuda/Cuda2.cu:1885-1904
+//   TnfPath::gen_tnf_graph     cuda/Cuda2.cu:861-970
+//   TnfPath::cal_tnf_dist      cuda/Cuda2.cu:721-779 (returns the distance)
+// Differences a caller should know:
+//   - state lives in a TnfPath object instead of globals (TNF_d, contig_log,
+//     nobs, maxEdges); device buffers are not freed by gen_tnf_graph;
+//   - edges are appended sorted by (from, to) (the reference's order depends
+//     on OpenMP scheduling);
+//   - on a library error the reference prints and exits (cuda/Cuda2.cu:
+//     341-354); TnfPath does the same unless exit_on_error is false, in which
+//     case it throws mdet::Error.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <cstdio>
+#include <cstdlib>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <utility>
+#include <vector>
+
+#include "tnf.h"
+
+namespace mdet {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+class Graph {
+   public:
+    size_t n;
+    std::vector<size_t> from;
+    std::vector<size_t> to;
+    std::vector<double> sTNF;
+    explicit Graph(size_t num_nodes) : n(num_nodes) {}
+    size_t getNodeCount() const { return n; }
+    size_t getEdgeCount() const { return from.size(); }
+    size_t getOtherNode(size_t e, size_t v) const { return from[e] == v ? to[e] : from[e]; }
+};
+
+class TnfPath {
+   public:
+    size_t maxEdges = 200;              // --maxEdges (cuda/Cuda2.cu:1265)
+    int precision = MDET_BF16_REFINE;   // prefilter arithmetic; all modes give the same edges
+    bool exit_on_error = true;
+
+    explicit TnfPath(int device = 0) { check(mdet_ctx_create(device, &ctx_), "mdet_ctx_create"); }
+    ~TnfPath() {
+        release();
+        mdet_ctx_destroy(ctx_);
+    }
+    TnfPath(const TnfPath&) = delete;
+    TnfPath& operator=(const TnfPath&) = delete;
+
+    size_t nobs() const { return nobs_; }
+    mdet_ctx* context() { return ctx_; }
+    const float* tnf_device() const { return tnf_; }
+    const double* logsize_device() const { return ls_; }
+
+    // TNF of the large contigs: packed on the host, counted on the device.
+    // logSizes = log10(min(len, 500000)) per contig as the reference's parser
+    // computes them (cuda/Cuda2.cu:1535); len counts every sequence byte.
+    void compute_tnf(const std::vector<std::string_view>& seqs) {
+        release();
+        nobs_ = seqs.size();
+        std::vector<uint64_t> off(nobs_ + 1, 0);
+        for (size_t r = 0; r < nobs_; ++r) off[r + 1] = off[r] + seqs[r].size();
+        std::string joined;
+        joined.reserve(off[nobs_]);
+        for (auto& s : seqs) joined.append(s.data(), s.size());
+        std::vector<uint32_t> packed((off[nobs_] + 15) / 16 + 1, 0);
+        uint64_t nr = 0, cap = off[nobs_] / 2 + 16;
+        std::vector<uint64_t> runs(2 * cap);
+        check(mdet_pack_ascii(joined.data(), off.data(), nobs_, packed.data(), packed.size(), runs.data(), cap, &nr),
+              "mdet_pack_ascii");
+        std::vector<uint64_t> lens(nobs_);
+        for (size_t r = 0; r < nobs_; ++r) lens[r] = off[r + 1] - off[r];
+        logSizes.resize(nobs_);
+        check(mdet_logsize_host(lens.data(), nobs_, logSizes.data()), "mdet_logsize_host");
+        void *dp = nullptr, *dr = nullptr, *doff = nullptr;
+        alloc(&dp, packed.size() * 4);
+        alloc(&dr, std::max<uint64_t>(2 * nr, 2) * 8);
+        alloc(&doff, off.size() * 8);
+        alloc(reinterpret_cast<void**>(&tnf_), nobs_ * MDET_NTNF * 4);
+        alloc(reinterpret_cast<void**>(&ls_), std::max<size_t>(nobs_, 1) * 8);
+        check(mdet_copy_h2d(ctx_, dp, packed.data(), packed.size() * 4), "copy");
+        if (nr) check(mdet_copy_h2d(ctx_, dr, runs.data(), 2 * nr * 8), "copy");
+        check(mdet_copy_h2d(ctx_, doff, off.data(), off.size() * 8), "copy");
+        check(mdet_copy_h2d(ctx_, ls_, logSizes.data(), nobs_ * 8), "copy");
+        check(mdet_tnf_count(ctx_, static_cast<uint32_t*>(dp), off[nobs_], static_cast<uint64_t*>(dr), nr,
+                             static_cast<uint64_t*>(doff), (uint32_t)nobs_, nullptr, tnf_),
+              "mdet_tnf_count");
+        check(mdet_ctx_sync(ctx_), "sync");
+        mdet_dev_free(ctx_, dp);
+        mdet_dev_free(ctx_, dr);
+        mdet_dev_free(ctx_, doff);
+    }
+
+    // Host copy of the TNF rows (the reference's float* TNF, cuda/Cuda2.cu:129).
+    std::vector<float> tnf_host() {
+        std::vector<float> h(nobs_ * MDET_NTNF);
+        if (!h.empty()) check(mdet_copy_d2h(ctx_, h.data(), tnf_, h.size() * 4), "copy");
+        return h;
+    }
+
+    // Use caller-provided TNF rows and log sizes (replaces the per-device
+    // copies of cuda/Cuda2.cu:1866-1874).
+    void set_tnf(const float* tnf_host_rows, const double* logsize, size_t nobs) {
+        release();
+        nobs_ = nobs;
+        logSizes.assign(logsize, logsize + nobs);
+        alloc(reinterpret_cast<void**>(&tnf_), nobs * MDET_NTNF * 4);
+        alloc(reinterpret_cast<void**>(&ls_), std::max<size_t>(nobs, 1) * 8);
+        check(mdet_copy_h2d(ctx_, tnf_, tnf_host_rows, nobs * MDET_NTNF * 4), "copy");
+        check(mdet_copy_h2d(ctx_, ls_, logsize, nobs * 8), "copy");
+    }
+
+    // gen_tnf_graph_sample (cuda/Cuda2.cu:972-1025): sample min(nobs, 2500)
+    // contigs (all if full) with the reference's partial Fisher-Yates on
+    // rand(), take each one's best similarity to any other contig, sort them
+    // descending, walk the per-mille cutoff p down from 999 with rand() steps
+    // until the fraction of sampled contigs reaching p reaches coverage.
+    size_t gen_tnf_graph_sample(double coverage = 1., bool full = false) {
+        const size_t _nobs = full ? nobs_ : std::min(nobs_, (size_t)2500);
+        std::vector<uint64_t> idx(nobs_);
+        std::iota(idx.begin(), idx.end(), 0);
+        {
+            size_t left = idx.size();
+            auto begin = idx.begin();
+            for (size_t k = 0; k < _nobs; ++k) {
+                auto r = begin + (rand() % left);
+                std::swap(*begin, *r);
+                ++begin;
+                --left;
+            }
+        }
+        std::vector<double> mx(_nobs);
+        void* didx = nullptr;
+        void* dmax = nullptr;
+        alloc(&didx, idx.size() * 8);
+        alloc(&dmax, std::max<size_t>(_nobs, 1) * 8);
+        check(mdet_copy_h2d(ctx_, didx, idx.data(), idx.size() * 8), "copy");
+        check(mdet_tnf_rowmax(ctx_, tnf_, ls_, (uint32_t)nobs_, static_cast<uint64_t*>(didx), (uint32_t)_nobs,
+                              static_cast<double*>(dmax)),
+              "mdet_tnf_rowmax");
+        check(mdet_copy_d2h(ctx_, mx.data(), dmax, _nobs * 8), "copy");
+        mdet_dev_free(ctx_, didx);
+        mdet_dev_free(ctx_, dmax);
+        std::sort(mx.begin(), mx.end(), std::greater<double>());
+        size_t p = 999, pp = 1000;
+        double cov = 0, pcov = 0;
+        size_t counton = 0;
+        while (p > 700) {
+            const double cutoff = (double)p / 1000.;
+            while (counton < _nobs && mx[counton] >= cutoff) counton++;
+            cov = _nobs ? (double)counton / _nobs : 0.0;
+            if (cov >= coverage) {
+                if (cov - coverage > coverage - pcov) {
+                    p = pp;
+                    cov = pcov;
+                }
+                break;
+            }
+            pp = p;
+            pcov = cov;
+            if (p > 990) p -= rand() % 3 + 1;
+            else if (p > 900) p -= rand() % 3 + 3;
+            else p -= rand() % 3 + 9;
+        }
+        return p;
+    }
+
+    // The pTNF decision of main() (cuda/Cuda2.cu:1885-1904): pTNF in percent
+    // from the command line (0 = automatic), maxP as a fraction; returns the
+    // per-mille value gen_tnf_graph's cutoff is pTNF/1000 of.
+    double select_ptnf(double pTNF, double maxP) {
+        if (pTNF >= 1.) return pTNF * 10;
+        if (nobs_ <= 25000) return (double)gen_tnf_graph_sample(maxP, true);
+        double acc = 0;
+        for (size_t i = 0; i < 10; ++i) {
+            double _minp = (double)gen_tnf_graph_sample(maxP);
+            if (_minp < 701) _minp = 700.;
+            acc += _minp;
+            if (i == 1 && acc / 2 < 701) return 700.;
+            if (i == 9) acc /= 10.;
+        }
+        return acc;
+    }
+
+    // gen_tnf_graph (cuda/Cuda2.cu:861-970): append every edge (i, j), i < j,
+    // with preProb(i,j) > log(1/(1-cutoff)-1) and j among row i's maxEdges
+    // most similar partners.
+    void gen_tnf_graph(Graph& g, double cutoff) {
+        mdet_edges e{};
+        check(mdet_tnf_edges(ctx_, tnf_, ls_, (uint32_t)nobs_, cutoff, (uint32_t)maxEdges, precision, &e, &last_stats),
+              "mdet_tnf_edges");
+        std::vector<uint32_t> f(e.count), t(e.count);
+        std::vector<float> s(e.count);
+        if (e.count) {
+            check(mdet_copy_d2h(ctx_, f.data(), e.from_dev, e.count * 4), "copy");
+            check(mdet_copy_d2h(ctx_, t.data(), e.to_dev, e.count * 4), "copy");
+            check(mdet_copy_d2h(ctx_, s.data(), e.s_dev, e.count * 4), "copy");
+        }
+        g.from.reserve(g.from.size() + e.count);
+        g.to.reserve(g.to.size() + e.count);
+        g.sTNF.reserve(g.sTNF.size() + e.count);
+        for (uint64_t k = 0; k < e.count; ++k) {
+            g.from.push_back(f[k]);
+            g.to.push_back(t[k]);
+            g.sTNF.push_back(s[k]);
+        }
+    }
+
+    // cal_tnf_dist (cuda/Cuda2.cu:721-779): the distance 1/(1+exp(preProb)),
+    // with the reference's floors, from the device FP64 evaluation.
+    double cal_tnf_dist(size_t r1, size_t r2) {
+        uint32_t pi = (uint32_t)r1, pj = (uint32_t)r2;
+        void *dpi = nullptr, *dpj = nullptr, *dp = nullptr;
+        alloc(&dpi, 4);
+        alloc(&dpj, 4);
+        alloc(&dp, 8);
+        check(mdet_copy_h2d(ctx_, dpi, &pi, 4), "copy");
+        check(mdet_copy_h2d(ctx_, dpj, &pj, 4), "copy");
+        check(mdet_tnf_preprob(ctx_, tnf_, ls_, (uint32_t)nobs_, static_cast<uint32_t*>(dpi),
+                               static_cast<uint32_t*>(dpj), 1, nullptr, static_cast<double*>(dp)),
+              "mdet_tnf_preprob");
+        double pre = 0;
+        check(mdet_copy_d2h(ctx_, &pre, dp, 8), "copy");
+        mdet_dev_free(ctx_, dpi);
+        mdet_dev_free(ctx_, dpj);
+        mdet_dev_free(ctx_, dp);
+        // the device value is model 1 when above ln 9, else model 2 clamped
+        // to <= ln 9; the host form maps the clamp to exactly 0.1
+        const double floor_pre = std::log(9.0);
+        return pre == floor_pre ? 0.1 : 1.0 / (1 + std::exp(pre));
+    }
+
+    std::vector<double> logSizes;
+    mdet_pair_stats last_stats{};
+
+   private:
+    void check(int rc, const char* what) {
+        if (rc == MDET_OK) return;
+        std::string msg = std::string(what) + ": " + mdet_last_error();
+        if (exit_on_error) {
+            std::fprintf(stderr, "Error: %s\n", msg.c_str());
+            std::exit(1);
+        }
+        throw Error(rc, msg);
+    }
+    void alloc(void** p, uint64_t bytes) { check(mdet_dev_alloc(ctx_, bytes, p), "mdet_dev_alloc"); }
+    void release() {
+        if (tnf_) mdet_dev_free(ctx_, tnf_);
+        if (ls_) mdet_dev_free(ctx_, ls_);
+        tnf_ = nullptr;
+        ls_ = nullptr;
+        nobs_ = 0;
+    }
+
+    mdet_ctx* ctx_ = nullptr;
+    float* tnf_ = nullptr;
+    double* ls_ = nullptr;
+    size_t nobs_ = 0;
+};
+
+}  // namespace mdet

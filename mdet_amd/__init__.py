@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdet_tnf.so")
 NTNF = 136
 
-EXACT_F32, BF16_REFINE, F16_REFINE = 0, 1, 2
+EXACT_F32, BF16_REFINE, F16_REFINE, LEARNED = 0, 1, 2, 3
 PRECISIONS = {"exact_f32": EXACT_F32, "bf16_refine": BF16_REFINE, "f16_refine": F16_REFINE}
 
 
@@ -76,6 +76,22 @@ _SIGS = {
     "mdet_tnf_count_ascii": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mdet_tnf_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double, C.c_uint32,
                                  C.c_int, C.POINTER(Edges), C.POINTER(PairStats)]),
+    "mdet_last_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]),
+    "mdet_mlp_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "mdet_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "mdet_mlp_param_count": (C.c_uint64, []),
+    "mdet_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mdet_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mdet_mlp_set_input_norm": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mdet_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float,
+                                C.c_void_p, C.c_void_p]),
+    "mdet_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_float, C.c_float, C.c_void_p]),
+    "mdet_mlp_embed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mdet_tnf_edges_learned": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double,
+                                         C.c_uint32, C.c_float, C.POINTER(Edges), C.POINTER(PairStats)]),
+    "mdet_tnf_rowmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                  C.c_void_p]),
     "mdet_tnf_preprob": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.c_uint64, C.c_void_p, C.c_void_p]),
 }
@@ -232,6 +248,53 @@ class DevArray:
             self.ptr = None
 
 
+class Mlp:
+    """The learned filter network (device-resident parameters)."""
+
+    def __init__(self, ctx: "Ctx", seed: int = 1):
+        h = C.c_void_p()
+        _check(lib().mdet_mlp_create(ctx.h, seed, C.byref(h)), "mdet_mlp_create")
+        self.h, self.ctx = h, ctx
+        self.n_params = int(lib().mdet_mlp_param_count())
+
+    def close(self):
+        if self.h:
+            lib().mdet_mlp_destroy(self.h)
+            self.h = None
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        _check(lib().mdet_mlp_get_params(self.h, _ptr(out), out.size), "mdet_mlp_get_params")
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().mdet_mlp_set_params(self.h, _ptr(p), p.size), "mdet_mlp_set_params")
+
+    def set_input_norm(self, mean: np.ndarray, istd: np.ndarray):
+        mean = np.ascontiguousarray(mean, np.float32)
+        istd = np.ascontiguousarray(istd, np.float32)
+        _check(lib().mdet_mlp_set_input_norm(self.h, _ptr(mean), _ptr(istd)), "mdet_mlp_set_input_norm")
+
+    def grad(self, tnf: "DevArray", pi: "DevArray", pj: "DevArray", y: "DevArray", batch: int, margin: float):
+        g = np.zeros(self.n_params, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(lib().mdet_mlp_grad(self.h, C.c_void_p(tnf.ptr), C.c_void_p(pi.ptr), C.c_void_p(pj.ptr),
+                                   C.c_void_p(y.ptr), batch, margin, _ptr(g), _ptr(loss)), "mdet_mlp_grad")
+        return float(loss[0]), g
+
+    def train_step(self, tnf, pi, pj, y, batch: int, lr: float, margin: float, want_loss: bool = False,
+                   offset: int = 0):
+        loss = np.zeros(1, np.float32)
+        _check(lib().mdet_mlp_train_step(self.h, C.c_void_p(tnf.ptr), C.c_void_p(pi.ptr + 4 * offset),
+                                         C.c_void_p(pj.ptr + 4 * offset), C.c_void_p(y.ptr + 4 * offset), batch,
+                                         lr, margin, _ptr(loss) if want_loss else None), "mdet_mlp_train_step")
+        return float(loss[0]) if want_loss else None
+
+    def embed(self, tnf: "DevArray", n: int, out: "DevArray"):
+        _check(lib().mdet_mlp_embed(self.h, C.c_void_p(tnf.ptr), n, C.c_void_p(out.ptr)), "mdet_mlp_embed")
+
+
 class Ctx:
     """One mdet context (device + stream + scratch)."""
 
@@ -283,11 +346,16 @@ class Ctx:
                                           C.c_void_p(tnf.ptr)), "mdet_tnf_count_ascii")
 
     def tnf_edges(self, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int,
-                  precision: int = EXACT_F32, fetch: bool = True):
+                  precision: int = EXACT_F32, fetch: bool = True, mlp: "Mlp | None" = None, radius2: float = 0.0):
         e = Edges()
         st = PairStats()
-        _check(lib().mdet_tnf_edges(self.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n, cutoff, max_edges,
-                                    precision, C.byref(e), C.byref(st)), "mdet_tnf_edges")
+        if precision == LEARNED:
+            _check(lib().mdet_tnf_edges_learned(self.h, mlp.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n,
+                                                cutoff, max_edges, radius2, C.byref(e), C.byref(st)),
+                   "mdet_tnf_edges_learned")
+        else:
+            _check(lib().mdet_tnf_edges(self.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n, cutoff, max_edges,
+                                        precision, C.byref(e), C.byref(st)), "mdet_tnf_edges")
         if not fetch:
             return e.count, st
         self.sync()
@@ -300,6 +368,28 @@ class Ctx:
             _check(lib().mdet_copy_d2h(self.h, _ptr(t), C.c_void_p(e.to_dev), 4 * k), "copy")
             _check(lib().mdet_copy_d2h(self.h, _ptr(s), C.c_void_p(e.s_dev), 4 * k), "copy")
         return f, t, s, st
+
+    def last_pairs(self) -> np.ndarray:
+        """Refine records of the last edge-list call: structured array with
+        fields i, j, pre, flags (bit 1: above threshold)."""
+        p, cnt = C.c_void_p(), C.c_uint64(0)
+        _check(lib().mdet_last_pairs(self.h, C.byref(p), C.byref(cnt)), "mdet_last_pairs")
+        dt = np.dtype([("i", np.uint32), ("j", np.uint32), ("pre", np.float32), ("flags", np.uint32)])
+        out = np.zeros(cnt.value, dt)
+        if cnt.value:
+            self.sync()
+            _check(lib().mdet_copy_d2h(self.h, _ptr(out), p, 16 * cnt.value), "copy")
+        return out
+
+    def tnf_rowmax(self, tnf: DevArray, logsize: DevArray, n: int, idx: np.ndarray, n_sample: int) -> np.ndarray:
+        d_idx = self.to_device(np.ascontiguousarray(idx, np.uint64))
+        out = self.empty(n_sample, np.float64)
+        _check(lib().mdet_tnf_rowmax(self.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n, C.c_void_p(d_idx.ptr),
+                                     n_sample, C.c_void_p(out.ptr)), "mdet_tnf_rowmax")
+        r = out.numpy()
+        d_idx.free()
+        out.free()
+        return r
 
     def tnf_preprob(self, tnf: DevArray, logsize: DevArray, n: int, pi: np.ndarray, pj: np.ndarray):
         pi_d = self.to_device(np.ascontiguousarray(pi, np.uint32))

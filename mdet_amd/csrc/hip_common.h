@@ -47,6 +47,7 @@ struct mdet_ctx {
     // cached MFMA work items for a padded row count
     uint32_t items_npad = 0;
     uint32_t nitems = 0;
+    uint64_t last_pairs = 0;  // refine records of the last edge-list call
 };
 
 namespace mdet {

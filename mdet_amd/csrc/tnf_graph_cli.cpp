@@ -1,0 +1,136 @@
+// mdet_tnf_graph: the TNF path of the reference's main() (cuda/Cuda2.cu:
+// 1251-1912) as a standalone program over the C++ drop-in: read contigs,
+// keep those >= minContig, TNF on the GPU, pTNF (automatic when 0), TNF
+// graph, edges written as "from<TAB>to<TAB>sTNF" (contig names).
+//
+//   mdet_tnf_graph -i contigs.fa -o edges.tsv [-m 2500] [--pTNF 0]
+//                  [--maxP 95] [--maxEdges 200] [--seed 0]
+//                  [--precision bf16|f16|exact] [--tnf tnf.f32]
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "mdet/tnf.hpp"
+
+namespace {
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+void usage() {
+    std::cerr << "usage: mdet_tnf_graph -i contigs.fa -o edges.tsv [-m minContig=2500] [--pTNF 0] [--maxP 95]\n"
+                 "                      [--maxEdges 200] [--seed 0] [--precision bf16|f16|exact] [--tnf out.f32]\n";
+}
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string in, out, tnf_out;
+    size_t minContig = 2500, maxEdges = 200;
+    double pTNF = 0, maxP = 95;
+    unsigned long long seed = 0;
+    int precision = MDET_BF16_REFINE;
+    for (int a = 1; a < argc; ++a) {
+        std::string k = argv[a];
+        auto val = [&]() -> std::string {
+            if (a + 1 >= argc) {
+                usage();
+                std::exit(1);
+            }
+            return argv[++a];
+        };
+        if (k == "-i" || k == "--inFile") in = val();
+        else if (k == "-o" || k == "--outFile") out = val();
+        else if (k == "-m" || k == "--minContig") minContig = std::stoull(val());
+        else if (k == "--pTNF") pTNF = std::stod(val());
+        else if (k == "--maxP") maxP = std::stod(val());
+        else if (k == "--maxEdges") maxEdges = std::stoull(val());
+        else if (k == "--seed") seed = std::stoull(val());
+        else if (k == "--tnf") tnf_out = val();
+        else if (k == "--precision") {
+            std::string v = val();
+            precision = v == "exact" ? MDET_EXACT_F32 : v == "f16" ? MDET_F16_REFINE : MDET_BF16_REFINE;
+        } else {
+            usage();
+            return 1;
+        }
+    }
+    if (in.empty() || out.empty()) {
+        usage();
+        return 1;
+    }
+    // argument checks of cuda/Cuda2.cu:1311-1332
+    if (seed == 0) seed = (unsigned long long)time(0);
+    srand((unsigned)seed);
+    if (maxP <= 0 || maxP >= 100) {
+        std::cerr << "[Error!] maxP should be greater than 0 and less than 100" << std::endl;
+        return 1;
+    }
+    if (pTNF < 0 || pTNF >= 100) {
+        std::cerr << "[Error!] pTNF should be >= 0 and < 100" << std::endl;
+        return 1;
+    }
+    if (minContig < 1500) {
+        std::cerr << "[Error!] Contig length < 1500 is not allowed to be used for binning." << std::endl;
+        return 1;
+    }
+    double t0 = now();
+    std::ifstream f(in, std::ios::binary);
+    if (!f) {
+        std::cerr << "[Error!] can't open the sequence fasta file " << in << std::endl;
+        return 1;
+    }
+    std::string buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    uint64_t nrec = 0, tot = 0;
+    if (mdet_fasta_scan(buf.data(), buf.size(), &nrec, &tot) != MDET_OK) {
+        std::cerr << "[Error!] " << mdet_last_error() << std::endl;
+        return 1;
+    }
+    std::vector<uint64_t> name_off(nrec), name_len(nrec), seq_off(nrec + 1);
+    std::string seqs(tot, '\0');
+    if (mdet_fasta_read(buf.data(), buf.size(), nrec, name_off.data(), name_len.data(), &seqs[0], seq_off.data()) !=
+        MDET_OK) {
+        std::cerr << "[Error!] " << mdet_last_error() << std::endl;
+        return 1;
+    }
+    std::vector<std::string_view> large;
+    std::vector<std::string> names;
+    for (uint64_t r = 0; r < nrec; ++r) {
+        const uint64_t len = seq_off[r + 1] - seq_off[r];
+        if (len >= minContig) {
+            large.emplace_back(seqs.data() + seq_off[r], len);
+            names.emplace_back(buf.data() + name_off[r], name_len[r]);
+        }
+    }
+    double t1 = now();
+    mdet::TnfPath path(0);
+    path.maxEdges = maxEdges;
+    path.precision = precision;
+    path.compute_tnf(large);
+    double t2 = now();
+    if (!tnf_out.empty()) {
+        std::vector<float> h = path.tnf_host();
+        std::ofstream o(tnf_out, std::ios::binary);
+        o.write(reinterpret_cast<const char*>(h.data()), h.size() * sizeof(float));
+    }
+    const double ptnf = path.select_ptnf(pTNF, maxP / 100.);
+    double t3 = now();
+    mdet::Graph g(path.nobs());
+    path.gen_tnf_graph(g, ptnf / 1000.);
+    double t4 = now();
+    std::ofstream o(out);
+    for (size_t e = 0; e < g.getEdgeCount(); ++e)
+        o << names[g.from[e]] << '\t' << names[g.to[e]] << '\t' << g.sTNF[e] << '\n';
+    std::fprintf(stderr,
+                 "%zu large contigs (>= %zu) of %llu; pTNF = %.1f; %zu edges\n"
+                 "# elapsed time (read): %.3fs\n# elapsed time (TNF_CAL): %.3fs\n"
+                 "# elapsed time (GRAPH_SAMPLE): %.3fs\n# elapsed time (TNF_GRAPH): %.3fs\n",
+                 path.nobs(), minContig, (unsigned long long)nrec, ptnf / 10., g.getEdgeCount(), t1 - t0, t2 - t1,
+                 t3 - t2, t4 - t3);
+    return 0;
+}

@@ -1,0 +1,423 @@
+// Learned candidate filter for the TNF graph: an MLP 136 -> 256 -> 256 -> 32
+// (GELU, BF16 weights, FP32 accumulation) whose embedding distance predicts
+// whether a contig pair can clear the edge threshold.  No reference
+// counterpart: the reference evaluates every pair.
+//
+// Training (all on the device, kernels below): a batch of labelled pairs
+// (i, j, y) gathers 2B input rows (standardised TNF), runs the forward pass
+// keeping pre-activations, evaluates the contrastive margin loss
+//     L = mean over pairs of  y * D + (1 - y) * relu(m - D),
+//     D = |e_i - e_j|^2,
+// back-propagates through the three layers and applies Adam to the FP32
+// master weights, refreshing their BF16 copies that the forward pass reads.
+//
+// GEMMs are FP32-accumulate tiles on the VALU (64x64 per 256-thread block,
+// 4x4 per thread, K streamed through LDS): the network is 109,088 weights and
+// a training step is ~10^9 flops, so these kernels are launch-bound, not
+// compute-bound; the all-pairs embedding Gram that uses the network at
+// inference runs on MFMA (tnf_pairs.hip).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hip_common.h"
+
+namespace {
+
+constexpr int D_IN = 136, D_H = 256, D_E = 32;
+constexpr int N_W1 = D_H * D_IN, N_W2 = D_H * D_H, N_W3 = D_E * D_H;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + N_W1, OFF_W2 = OFF_B1 + D_H, OFF_B2 = OFF_W2 + N_W2, OFF_W3 = OFF_B2 + D_H,
+              OFF_B3 = OFF_W3 + N_W3, N_PARAMS = OFF_B3 + D_E;  // 109,088
+
+__device__ inline float bf16_round(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return f;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+__device__ inline float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float x) {
+    const float cdf = 0.5f * (1.f + erff(x * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+    return cdf + x * pdf;
+}
+
+// C[M x N] (+)= op(A) * op(B); A is M x K (row-major, lda) or K x M if TA;
+// B is K x N (row-major, ldb) or N x K if TB.  Epilogue: + bias[n], store
+// pre-activation to Z (optional) and act(.) to C (GELU if gelu_out).
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, int lda, const float* __restrict__ B,
+                                              int ldb, float* __restrict__ C, int ldc, float* __restrict__ Z, int M,
+                                              int N, int K, const float* __restrict__ bias, int gelu_out,
+                                              int accumulate) {
+    __shared__ float As[16][64 + 1], Bs[16][64 + 1];
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    float acc[4][4] = {};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        for (int e = tid; e < 16 * 64; e += 256) {
+            const int kk = e >> 6, mm = e & 63;
+            const int m = m0 + mm, k = k0 + kk;
+            float va = 0.f, vb = 0.f;
+            if (m < M && k < K) va = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+            const int n = n0 + mm;
+            if (n < N && k < K) vb = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
+            As[kk][mm] = va;
+            Bs[kk][mm] = vb;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + ty + 16 * i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + tx + 16 * j;
+            if (n >= N) continue;
+            float v = acc[i][j] + (bias ? bias[n] : 0.f);
+            if (accumulate) v += C[(size_t)m * ldc + n];
+            if (Z) Z[(size_t)m * ldc + n] = v;
+            C[(size_t)m * ldc + n] = gelu_out ? gelu(v) : v;
+        }
+    }
+}
+
+// Standardised inputs: rows 2b and 2b+1 of X are pairs[b]'s two contigs.
+__global__ void k_gather_pairs(const float* __restrict__ tnf, const uint32_t* __restrict__ pi,
+                               const uint32_t* __restrict__ pj, uint32_t B, const float* __restrict__ mean,
+                               const float* __restrict__ istd, float* __restrict__ X) {
+    const uint32_t r = blockIdx.x, k = threadIdx.x;
+    if (r >= 2 * B || k >= D_IN) return;
+    const uint32_t c = (r & 1) ? pj[r >> 1] : pi[r >> 1];
+    float v = tnf[(size_t)c * D_IN + k];
+    if (v != v) v = 0.f;  // rows with no valid window
+    X[(size_t)r * D_IN + k] = (v - mean[k]) * istd[k];
+}
+
+__global__ void k_gather_rows(const float* __restrict__ tnf, uint32_t n, const float* __restrict__ mean,
+                              const float* __restrict__ istd, float* __restrict__ X) {
+    const uint32_t r = blockIdx.x, k = threadIdx.x;
+    if (r >= n || k >= D_IN) return;
+    float v = tnf[(size_t)r * D_IN + k];
+    if (v != v) v = 0.f;
+    X[(size_t)r * D_IN + k] = (v - mean[k]) * istd[k];
+}
+
+// Loss and dL/dE for each pair (one wave per pair, lanes = 32 embedding
+// dims, 2 pairs per 64-wide wave).  loss_sum gets the batch sum.
+__global__ void k_pair_loss(const float* __restrict__ E, const float* __restrict__ y, uint32_t B, float margin,
+                            float* __restrict__ dE, float* __restrict__ loss_sum) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t b = g >> 5, d = g & 31;
+    float diff = 0.f;
+    if (b < B) diff = E[(size_t)(2 * b) * D_E + d] - E[(size_t)(2 * b + 1) * D_E + d];
+    float D = diff * diff;
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) D += __shfl_xor(D, off, 32);
+    if (b >= B) return;
+    const float lab = y[b];
+    const float l = lab * D + (1.f - lab) * fmaxf(margin - D, 0.f);
+    const float dldD = (lab + (1.f - lab) * (D < margin ? -1.f : 0.f)) / (float)B;
+    dE[(size_t)(2 * b) * D_E + d] = dldD * 2.f * diff;
+    dE[(size_t)(2 * b + 1) * D_E + d] = -dldD * 2.f * diff;
+    if (d == 0) atomicAdd(loss_sum, l / (float)B);
+}
+
+// dZ = dA * gelu'(Z)
+__global__ void k_gelu_back(const float* __restrict__ dA, const float* __restrict__ Z, float* __restrict__ dZ,
+                            uint64_t count) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) dZ[k] = dA[k] * gelu_grad(Z[k]);
+}
+
+// db[n] = sum_m dZ[m][n]
+__global__ void k_colsum(const float* __restrict__ dZ, int M, int N, float* __restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int m = 0; m < M; ++m) s += dZ[(size_t)m * N + n];
+    db[n] = s;
+}
+
+// Adam on the FP32 master copy; writes the BF16-rounded weights the forward
+// pass reads (biases stay FP32).
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ wq, const float* __restrict__ g,
+                       float* __restrict__ m1, float* __restrict__ m2, uint32_t count, float lr, float b1, float b2,
+                       float eps, float c1, float c2) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const float gk = g[k];
+    const float a = b1 * m1[k] + (1.f - b1) * gk;
+    const float v = b2 * m2[k] + (1.f - b2) * gk * gk;
+    m1[k] = a;
+    m2[k] = v;
+    const float w = p[k] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+    p[k] = w;
+    const bool is_bias = (k >= OFF_B1 && k < OFF_W2) || (k >= OFF_B2 && k < OFF_W3) || k >= OFF_B3;
+    wq[k] = is_bias ? w : bf16_round(w);
+}
+
+__global__ void k_quantize(const float* __restrict__ p, float* __restrict__ wq, uint32_t count) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const bool is_bias = (k >= OFF_B1 && k < OFF_W2) || (k >= OFF_B2 && k < OFF_W3) || k >= OFF_B3;
+    wq[k] = is_bias ? p[k] : bf16_round(p[k]);
+}
+
+}  // namespace
+
+struct mdet_mlp {
+    mdet_ctx* ctx = nullptr;
+    float* p = nullptr;   // FP32 master parameters
+    float* wq = nullptr;  // parameters as the forward pass uses them (BF16-rounded weights)
+    float* g = nullptr;   // gradient
+    float* m1 = nullptr;
+    float* m2 = nullptr;
+    float* norm = nullptr;  // mean[136], inv std[136]
+    uint64_t step = 0;
+    // activations of the last batch (rows = 2 * batch)
+    std::vector<void*> owned;
+};
+
+namespace {
+
+using namespace mdet;
+
+int mlp_alloc(mdet_mlp* m, size_t bytes, void** out) {
+    hipError_t e = hipMalloc(out, bytes ? bytes : 4);
+    if (e != hipSuccess) MDET_FAIL(MDET_E_NOMEM, "mlp: hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    m->owned.push_back(*out);
+    return MDET_OK;
+}
+
+dim3 gemm_grid(int M, int N) { return dim3((N + 63) / 64, (M + 63) / 64); }
+
+// forward for M rows of X (standardised input); writes Z1,A1,Z2,A2,E
+int mlp_forward(mdet_mlp* m, const float* X, int M, float* Z1, float* A1, float* Z2, float* A2, float* E) {
+    hipStream_t s = m->ctx->stream;
+    const float* w = m->wq;
+    hipLaunchKernelGGL((k_gemm<false, true>), gemm_grid(M, D_H), dim3(256), 0, s, X, D_IN, w + OFF_W1, D_IN, A1, D_H,
+                       Z1, M, D_H, D_IN, w + OFF_B1, 1, 0);
+    hipLaunchKernelGGL((k_gemm<false, true>), gemm_grid(M, D_H), dim3(256), 0, s, A1, D_H, w + OFF_W2, D_H, A2, D_H, Z2,
+                       M, D_H, D_H, w + OFF_B2, 1, 0);
+    hipLaunchKernelGGL((k_gemm<false, true>), gemm_grid(M, D_E), dim3(256), 0, s, A2, D_H, w + OFF_W3, D_H, E, D_E,
+                       (float*)nullptr, M, D_E, D_H, w + OFF_B3, 0, 0);
+    MDET_HIP(hipGetLastError());
+    return MDET_OK;
+}
+
+struct Scratch {
+    float *X, *Z1, *A1, *Z2, *A2, *E, *dE, *dA2, *dZ2, *dA1, *dZ1, *loss;
+};
+
+int scratch(mdet_mlp* m, int M, Scratch* S) {
+    mdet_ctx* c = m->ctx;
+    MDET_TRY(ctx_buf(c, "mlp.X", (size_t)M * D_IN, &S->X));
+    MDET_TRY(ctx_buf(c, "mlp.Z1", (size_t)M * D_H, &S->Z1));
+    MDET_TRY(ctx_buf(c, "mlp.A1", (size_t)M * D_H, &S->A1));
+    MDET_TRY(ctx_buf(c, "mlp.Z2", (size_t)M * D_H, &S->Z2));
+    MDET_TRY(ctx_buf(c, "mlp.A2", (size_t)M * D_H, &S->A2));
+    MDET_TRY(ctx_buf(c, "mlp.E", (size_t)M * D_E, &S->E));
+    MDET_TRY(ctx_buf(c, "mlp.dE", (size_t)M * D_E, &S->dE));
+    MDET_TRY(ctx_buf(c, "mlp.dA2", (size_t)M * D_H, &S->dA2));
+    MDET_TRY(ctx_buf(c, "mlp.dZ2", (size_t)M * D_H, &S->dZ2));
+    MDET_TRY(ctx_buf(c, "mlp.dA1", (size_t)M * D_H, &S->dA1));
+    MDET_TRY(ctx_buf(c, "mlp.dZ1", (size_t)M * D_H, &S->dZ1));
+    MDET_TRY(ctx_buf(c, "mlp.loss", 4, &S->loss));
+    return MDET_OK;
+}
+
+// loss + gradient (into m->g) of one batch of pairs
+int mlp_grad(mdet_mlp* m, const float* tnf, const uint32_t* pi, const uint32_t* pj, const float* y, uint32_t B,
+             float margin, Scratch* S) {
+    hipStream_t s = m->ctx->stream;
+    const int M = 2 * (int)B;
+    MDET_TRY(scratch(m, M, S));
+    MDET_HIP(hipMemsetAsync(S->loss, 0, 4, s));
+    hipLaunchKernelGGL(k_gather_pairs, dim3(M), dim3(160), 0, s, tnf, pi, pj, B, m->norm, m->norm + D_IN, S->X);
+    MDET_TRY(mlp_forward(m, S->X, M, S->Z1, S->A1, S->Z2, S->A2, S->E));
+    hipLaunchKernelGGL(k_pair_loss, dim3((B * 32 + 255) / 256), dim3(256), 0, s, S->E, y, B, margin, S->dE, S->loss);
+    const float* w = m->wq;
+    float* g = m->g;
+    // layer 3: dW3 = dE^T A2, db3 = sum dE, dA2 = dE W3
+    hipLaunchKernelGGL((k_gemm<true, false>), gemm_grid(D_E, D_H), dim3(256), 0, s, S->dE, D_E, S->A2, D_H,
+                       g + OFF_W3, D_H, (float*)nullptr, D_E, D_H, M, (const float*)nullptr, 0, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, s, S->dE, M, D_E, g + OFF_B3);
+    hipLaunchKernelGGL((k_gemm<false, false>), gemm_grid(M, D_H), dim3(256), 0, s, S->dE, D_E, w + OFF_W3, D_H, S->dA2,
+                       D_H, (float*)nullptr, M, D_H, D_E, (const float*)nullptr, 0, 0);
+    const uint64_t cnt = (uint64_t)M * D_H;
+    hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, S->dA2, S->Z2, S->dZ2, cnt);
+    // layer 2
+    hipLaunchKernelGGL((k_gemm<true, false>), gemm_grid(D_H, D_H), dim3(256), 0, s, S->dZ2, D_H, S->A1, D_H,
+                       g + OFF_W2, D_H, (float*)nullptr, D_H, D_H, M, (const float*)nullptr, 0, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(4), dim3(64), 0, s, S->dZ2, M, D_H, g + OFF_B2);
+    hipLaunchKernelGGL((k_gemm<false, false>), gemm_grid(M, D_H), dim3(256), 0, s, S->dZ2, D_H, w + OFF_W2, D_H,
+                       S->dA1, D_H, (float*)nullptr, M, D_H, D_H, (const float*)nullptr, 0, 0);
+    hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, S->dA1, S->Z1, S->dZ1, cnt);
+    // layer 1
+    hipLaunchKernelGGL((k_gemm<true, false>), gemm_grid(D_H, D_IN), dim3(256), 0, s, S->dZ1, D_H, S->X, D_IN,
+                       g + OFF_W1, D_IN, (float*)nullptr, D_H, D_IN, M, (const float*)nullptr, 0, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(4), dim3(64), 0, s, S->dZ1, M, D_H, g + OFF_B1);
+    MDET_HIP(hipGetLastError());
+    return MDET_OK;
+}
+
+// Xavier-uniform weights from a splitmix64 stream, zero biases.
+void init_params(std::vector<float>& p, uint64_t seed) {
+    uint64_t x = seed ^ 0x5DEECE66Dull;
+    auto next = [&]() {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    p.assign(N_PARAMS, 0.f);
+    auto fill = [&](int off, int fan_out, int fan_in) {
+        const double a = std::sqrt(6.0 / (fan_in + fan_out));
+        for (int k = 0; k < fan_out * fan_in; ++k) p[off + k] = (float)((2 * next() - 1) * a);
+    };
+    fill(OFF_W1, D_H, D_IN);
+    fill(OFF_W2, D_H, D_H);
+    fill(OFF_W3, D_E, D_H);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdet_mlp_create(mdet_ctx* ctx, uint64_t seed, mdet_mlp** out) {
+    MDET_TRY(check_ctx(ctx));
+    if (!out) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_create: null output");
+    mdet_mlp* m = new mdet_mlp();
+    m->ctx = ctx;
+    int rc = MDET_OK;
+    void* v;
+    if ((rc = mlp_alloc(m, N_PARAMS * 4, &v)) == MDET_OK) m->p = (float*)v;
+    if (!rc && (rc = mlp_alloc(m, N_PARAMS * 4, &v)) == MDET_OK) m->wq = (float*)v;
+    if (!rc && (rc = mlp_alloc(m, N_PARAMS * 4, &v)) == MDET_OK) m->g = (float*)v;
+    if (!rc && (rc = mlp_alloc(m, N_PARAMS * 4, &v)) == MDET_OK) m->m1 = (float*)v;
+    if (!rc && (rc = mlp_alloc(m, N_PARAMS * 4, &v)) == MDET_OK) m->m2 = (float*)v;
+    if (!rc && (rc = mlp_alloc(m, 2 * D_IN * 4, &v)) == MDET_OK) m->norm = (float*)v;
+    if (rc) {
+        mdet_mlp_destroy(m);
+        return rc;
+    }
+    std::vector<float> p;
+    init_params(p, seed);
+    std::vector<float> norm(2 * D_IN, 0.f);
+    for (int k = 0; k < D_IN; ++k) norm[D_IN + k] = 1.f;
+    MDET_HIP(hipMemcpy(m->p, p.data(), N_PARAMS * 4, hipMemcpyHostToDevice));
+    MDET_HIP(hipMemcpy(m->norm, norm.data(), 2 * D_IN * 4, hipMemcpyHostToDevice));
+    MDET_HIP(hipMemset(m->m1, 0, N_PARAMS * 4));
+    MDET_HIP(hipMemset(m->m2, 0, N_PARAMS * 4));
+    hipLaunchKernelGGL(k_quantize, dim3((N_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, m->p, m->wq,
+                       (uint32_t)N_PARAMS);
+    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    *out = m;
+    return MDET_OK;
+}
+
+int mdet_mlp_destroy(mdet_mlp* m) {
+    if (!m) return MDET_OK;
+    if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
+    for (void* p : m->owned) (void)hipFree(p);
+    delete m;
+    return MDET_OK;
+}
+
+uint64_t mdet_mlp_param_count(void) { return N_PARAMS; }
+
+int mdet_mlp_set_input_norm(mdet_mlp* m, const float* mean136, const float* istd136) {
+    if (!m || !mean136 || !istd136) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_set_input_norm: null argument");
+    MDET_HIP(hipMemcpy(m->norm, mean136, D_IN * 4, hipMemcpyHostToDevice));
+    MDET_HIP(hipMemcpy(m->norm + D_IN, istd136, D_IN * 4, hipMemcpyHostToDevice));
+    return MDET_OK;
+}
+
+int mdet_mlp_get_params(mdet_mlp* m, float* out, uint64_t count) {
+    if (!m || !out || count < N_PARAMS) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_get_params: need %d floats", N_PARAMS);
+    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
+    MDET_HIP(hipMemcpy(out, m->p, N_PARAMS * 4, hipMemcpyDeviceToHost));
+    return MDET_OK;
+}
+
+int mdet_mlp_set_params(mdet_mlp* m, const float* in, uint64_t count) {
+    if (!m || !in || count != N_PARAMS) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_set_params: need %d floats", N_PARAMS);
+    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
+    MDET_HIP(hipMemcpy(m->p, in, N_PARAMS * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_quantize, dim3((N_PARAMS + 255) / 256), dim3(256), 0, m->ctx->stream, m->p, m->wq,
+                       (uint32_t)N_PARAMS);
+    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
+    return MDET_OK;
+}
+
+int mdet_mlp_grad(mdet_mlp* m, const float* tnf_dev, const uint32_t* pi_dev, const uint32_t* pj_dev,
+                  const float* label_dev, uint32_t batch, float margin, float* grad_out, float* loss_out) {
+    if (!m) MDET_FAIL(MDET_E_INVALID, "null mdet_mlp");
+    MDET_TRY(check_ctx(m->ctx));
+    if (batch == 0 || !tnf_dev || !pi_dev || !pj_dev || !label_dev)
+        MDET_FAIL(MDET_E_INVALID, "mdet_mlp_grad: empty batch or null argument");
+    Scratch S;
+    MDET_TRY(mlp_grad(m, tnf_dev, pi_dev, pj_dev, label_dev, batch, margin, &S));
+    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
+    if (grad_out) MDET_HIP(hipMemcpy(grad_out, m->g, N_PARAMS * 4, hipMemcpyDeviceToHost));
+    if (loss_out) MDET_HIP(hipMemcpy(loss_out, S.loss, 4, hipMemcpyDeviceToHost));
+    return MDET_OK;
+}
+
+int mdet_mlp_train_step(mdet_mlp* m, const float* tnf_dev, const uint32_t* pi_dev, const uint32_t* pj_dev,
+                        const float* label_dev, uint32_t batch, float lr, float margin, float* loss_out) {
+    if (!m) MDET_FAIL(MDET_E_INVALID, "null mdet_mlp");
+    MDET_TRY(check_ctx(m->ctx));
+    if (batch == 0 || !tnf_dev || !pi_dev || !pj_dev || !label_dev)
+        MDET_FAIL(MDET_E_INVALID, "mdet_mlp_train_step: empty batch or null argument");
+    Scratch S;
+    MDET_TRY(mlp_grad(m, tnf_dev, pi_dev, pj_dev, label_dev, batch, margin, &S));
+    m->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - (float)std::pow((double)b1, (double)m->step);
+    const float c2 = 1.f - (float)std::pow((double)b2, (double)m->step);
+    hipLaunchKernelGGL(k_adam, dim3((N_PARAMS + 255) / 256), dim3(256), 0, m->ctx->stream, m->p, m->wq, m->g, m->m1,
+                       m->m2, (uint32_t)N_PARAMS, lr, b1, b2, 1e-8f, c1, c2);
+    MDET_HIP(hipGetLastError());
+    if (loss_out) {
+        MDET_HIP(hipMemcpyAsync(loss_out, S.loss, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+        MDET_HIP(hipStreamSynchronize(m->ctx->stream));
+    }
+    return MDET_OK;
+}
+
+int mdet_mlp_embed(mdet_mlp* m, const float* tnf_dev, uint32_t n, float* emb_dev) {
+    if (!m) MDET_FAIL(MDET_E_INVALID, "null mdet_mlp");
+    MDET_TRY(check_ctx(m->ctx));
+    if (n == 0) return MDET_OK;
+    if (!tnf_dev || !emb_dev) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_embed: null argument");
+    const int CH = 65536;  // rows per pass
+    for (uint32_t r0 = 0; r0 < n; r0 += CH) {
+        const int M = (int)std::min<uint32_t>(CH, n - r0);
+        Scratch S;
+        MDET_TRY(scratch(m, M, &S));
+        hipLaunchKernelGGL(k_gather_rows, dim3(M), dim3(160), 0, m->ctx->stream, tnf_dev + (size_t)r0 * D_IN,
+                           (uint32_t)M, m->norm, m->norm + D_IN, S.X);
+        MDET_TRY(mlp_forward(m, S.X, M, S.Z1, S.A1, S.Z2, S.A2, emb_dev + (size_t)r0 * D_E));
+    }
+    return MDET_OK;
+}
+
+}  // extern "C"

@@ -31,11 +31,13 @@
 
 #include "hip_common.h"
 #include "model_tables.h"
+#include "tnf_tables.h"
 
 namespace {
 
 constexpr int NT = 136;
-constexpr int KS = 9;  // MFMA k-steps of 16 (K padded to 144)
+constexpr int KS = 9;    // MFMA k-steps of 16 for TNF rows (K padded to 144)
+constexpr int KS_E = 3;  // k-steps for 32-dimensional embeddings (K padded to 48)
 constexpr float FLOOR32 = (float)MODEL_FLOOR_PREPROB;
 // Bound on |preProb_fp32 - preProb_fp64| used for the tie band.  Measured
 // errors of the FP32 evaluator are ~1e-5 (tests check < EPS/8).
@@ -317,6 +319,52 @@ __device__ inline uint16_t to_bf16_rne(float f) {
 }
 __device__ inline float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Locality order for the MFMA prefilter.  Contigs are visited in order of a
+// composition key (the GC content implied by the TNF row: sum over columns of
+// the column's G+C fraction times its frequency), so that contigs of one
+// genome sit close together: survivors then fall into a band around the
+// diagonal, most 32x32 tiles have none (fast path only), and the tiles that
+// have survivors are dense.  Every pair is still evaluated; results are
+// mapped back through the permutation.
+struct GcWeights {
+    float w[NT];
+};
+constexpr GcWeights make_gc_weights() {
+    GcWeights g{};
+    for (int c = 0; c < NT; ++c) {
+        int gcn = 0;
+        for (int k = 0; k < 4; ++k) {
+            const char ch = mdet::kTnColumnsCx[4 * c + k];
+            gcn += (ch == 'G' || ch == 'C') ? 1 : 0;
+        }
+        g.w[c] = 0.25f * (float)gcn;
+    }
+    return g;
+}
+constexpr GcWeights kGc = make_gc_weights();
+#define W8(i) kGc.w[i], kGc.w[i + 1], kGc.w[i + 2], kGc.w[i + 3], kGc.w[i + 4], kGc.w[i + 5], kGc.w[i + 6], kGc.w[i + 7]
+__constant__ float c_gcw[NT] = {W8(0),  W8(8),  W8(16), W8(24), W8(32),  W8(40),  W8(48),  W8(56), W8(64),
+                                W8(72), W8(80), W8(88), W8(96), W8(104), W8(112), W8(120), W8(128)};
+#undef W8
+
+// One wavefront per row: key = sum_c gcw[c] * tnf[c]; idx = row.  Rows whose
+// key is NaN (no valid window) sort last.
+__global__ __launch_bounds__(256) void k_sort_key(const float* __restrict__ tnf, uint32_t n, float* __restrict__ key,
+                                                  uint32_t* __restrict__ idx) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const float* t = tnf + (size_t)r * NT;
+    float v = c_gcw[lane] * t[lane] + c_gcw[lane + 64] * t[lane + 64];
+    if (lane < 8) v += c_gcw[lane + 128] * t[lane + 128];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) {
+        key[r] = (v == v) ? v : __builtin_inff();
+        idx[r] = r;
+    }
+}
+
 // Low-precision rows in MFMA fragment order.  For the 32-row tile t, k-step
 // ks and lane l (row t*32 + l%32, k = 16*ks + 8*(l/32) .. +8), the 16 bytes
 // sit at element offset ((t*9 + ks)*64 + l)*8: one coalesced 1 KiB load per
@@ -350,55 +398,62 @@ __device__ inline uint4 pack8(const uint16_t* h) {
     return w;
 }
 
-template <bool BF16>
-__global__ __launch_bounds__(256) void k_prep_lowp(const float* __restrict__ tnf, uint32_t n, uint32_t n_pad, float R2,
-                                                   uint4* __restrict__ frag, uint4* __restrict__ frag_a8) {
+// Rows of D values (D = 136 TNF columns, or 32 embedding dimensions) padded
+// to KSX k-steps of 16; the threshold slots are k = D .. D+5.
+template <bool BF16, int D, int KSX>
+__global__ __launch_bounds__(256) void k_prep_lowp(const float* __restrict__ x, const uint32_t* __restrict__ perm,
+                                                   uint32_t n, uint32_t n_pad, float R2, uint4* __restrict__ frag,
+                                                   uint4* __restrict__ frag_a) {
+    static_assert(D + 6 <= 16 * KSX, "no room for the threshold slots");
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= n_pad) return;
     const uint32_t t = r >> 5;
+    const size_t src = (r < n) ? (size_t)perm[r] : 0;  // sorted position r holds row perm[r]
+    const int k0 = 8 * (int)lane;
+    float vals[8];
     float part = 0.f;
-    uint16_t h[8];
-    if (lane < 17) {
-        const int k0 = 8 * lane;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float v = (r < n) ? tnf[(size_t)r * NT + k0 + e] : 0.f;
-            h[e] = to_low<BF16>(v);
-            const float back = from_low<BF16>(h[e]);
-            part = fmaf(back, back, part);
-        }
-        const uint32_t ks = lane >> 1, half = lane & 1;
-        frag[((size_t)t * KS + ks) * 64 + (r & 31) + 32 * half] = pack8(h);
+    for (int e = 0; e < 8; ++e) {
+        const int k = k0 + e;
+        const float v = (r < n && k < D && lane < 2 * KSX) ? x[src * D + k] : 0.f;
+        const float back = from_low<BF16>(to_low<BF16>(v));
+        vals[e] = v;
+        part = fmaf(back, back, part);
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
-    if (lane == 0) {
-        // padding rows get a large finite h (the split of +inf would be NaN)
-        const float big = BF16 ? 1e30f : 6.0e4f;
-        float hv = (r < n) ? fmaf(0.5f, part, -0.25f * R2) : big;
-        uint16_t p[3];
-        float rem = hv;
+    if (lane >= 2 * KSX) return;
+    // padding rows get a large finite h (the split of +inf would be NaN)
+    const float big = BF16 ? 1e30f : 6.0e4f;
+    const float hv = (r < n) ? fmaf(0.5f, part, -0.25f * R2) : big;
+    uint16_t hp[3];
+    float rem = hv;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            p[q] = to_low<BF16>(rem);
-            rem = rem - from_low<BF16>(p[q]);
-        }
-        const uint16_t one = to_low<BF16>(1.f);
-        uint16_t nb[8], na[8];
-        for (int q = 0; q < 3; ++q) {
-            nb[q] = one;
-            nb[3 + q] = p[q] ^ 0x8000u;  // negate
-            na[q] = p[q] ^ 0x8000u;
-            na[3 + q] = one;
-        }
-        nb[6] = nb[7] = na[6] = na[7] = 0;
-        frag[((size_t)t * KS + 8) * 64 + (r & 31) + 32] = pack8(nb);
-        frag_a8[(size_t)t * 64 + (r & 31) + 32] = pack8(na);
-    } else if (lane == 16) {
-        // the first half of k-step 8 (k = 128..135) is plain TNF in both roles
-        frag_a8[(size_t)t * 64 + (r & 31)] = pack8(h);
+    for (int q = 0; q < 3; ++q) {
+        hp[q] = to_low<BF16>(rem);
+        rem = rem - from_low<BF16>(hp[q]);
     }
+    const uint16_t one = to_low<BF16>(1.f);
+    uint16_t hb[8], ha[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = k0 + e;
+        uint16_t v = (k < D) ? to_low<BF16>(vals[e]) : (uint16_t)0;
+        uint16_t va = v;
+        if (k >= D && k < D + 3) {  // column role: 1, row role: -h pieces
+            v = one;
+            va = hp[k - D] ^ 0x8000u;
+        } else if (k >= D + 3 && k < D + 6) {  // column role: -h pieces, row role: 1
+            v = hp[k - D - 3] ^ 0x8000u;
+            va = one;
+        }
+        hb[e] = v;
+        ha[e] = va;
+    }
+    const uint32_t ks = lane >> 1, half = lane & 1;
+    frag[((size_t)t * KSX + ks) * 64 + (r & 31) + 32 * half] = pack8(hb);
+    if (ks == KSX - 1) frag_a[(size_t)t * 64 + (r & 31) + 32 * half] = pack8(ha);
 }
 
 // ------------------------------------------------------ FP32 prefilter ----
@@ -488,7 +543,6 @@ __global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__
 // steps with survivors enter the extraction pass, whose cost scales with the
 // number of (tile, accumulator) slots that hold survivors.
 constexpr int M_ROWS = 256, M_STEP = 128, M_SEG = 2048;
-constexpr int M_STEP_U4 = M_STEP / 32 * KS * 64;  // uint4 per B step (2304)
 
 template <bool BF16>
 __device__ inline f32x16_t mfma32(uint4 a, uint4 b, f32x16_t c) {
@@ -499,44 +553,68 @@ __device__ inline f32x16_t mfma32(uint4 a, uint4 b, f32x16_t c) {
                                                   0, 0);
 }
 
+template <int KSX>
 __device__ inline void issue_btile(const uint4* __restrict__ frag, uint32_t c0, uint4* lds, int tid, int wave) {
-    const uint4* src = frag + (size_t)(c0 / 32) * KS * 64;
+    const uint4* src = frag + (size_t)(c0 / 32) * KSX * 64;
 #pragma unroll
-    for (int m = 0; m < KS; ++m)
+    for (int m = 0; m < KSX; ++m)
         __builtin_amdgcn_global_load_lds((const void*)(src + 256 * m + tid),
                                          (__attribute__((address_space(3))) void*)(lds + 256 * m + 64 * wave), 16, 0,
                                          0);
 }
 
-template <bool BF16>
+// Per-wave survivor staging in LDS: survivors of a step are appended to a
+// 128-entry wave-private buffer and written out (one shard atomic + one
+// coalesced copy) only when it fills or the item ends, so most steps issue
+// neither an atomic nor a global store.
+constexpr int M_STAGE = 128;
+
+__device__ inline void stage_flush(const SurvOut& o, uint32_t shard, uint2* stage, uint32_t& n, int lane) {
+    if (n == 0) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&o.cnt[(size_t)shard * SHARD_STRIDE], (unsigned long long)n);
+    base = __shfl(base, 0, 64);
+    for (uint32_t k = lane; k < n; k += 64) {
+        const uint2 v = stage[k];
+        shard_put(o, shard, base + k, v.x, v.y);
+    }
+    n = 0;
+}
+
+template <bool BF16, int KSX>
 __global__ __launch_bounds__(256, 2) void k_prefilter_mfma(const uint4* __restrict__ frag,
                                                            const uint4* __restrict__ frag_a8,
                                                            const uint4* __restrict__ items, uint32_t n,
                                                            SurvOut out) {
-    __shared__ uint4 Bl[2][M_STEP_U4];
+    constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;  // uint4 per B step
+    __shared__ uint4 Bl[2][STEP_U4];
+    __shared__ uint2 stage_all[4][M_STAGE];
     const uint4 it = items[blockIdx.x];
     const uint32_t rb = it.x, c_begin = it.y, c_end = it.z;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t row0 = rb * M_ROWS + wave * 64;  // this wave's 64 rows
     const uint32_t t0 = row0 >> 5;
+    uint2* stage = stage_all[wave];
+    uint32_t nstage = 0;
 
-    issue_btile(frag, c_begin, Bl[0], tid, wave);
-    uint4 a[2][KS];
+    issue_btile<KSX>(frag, c_begin, Bl[0], tid, wave);
+    uint4 a[2][KSX];
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) {
 #pragma unroll
-        for (int ks = 0; ks < KS - 1; ++ks) a[rt][ks] = frag[((size_t)(t0 + rt) * KS + ks) * 64 + lane];
-        a[rt][KS - 1] = frag_a8[(size_t)(t0 + rt) * 64 + lane];
+        for (int ks = 0; ks < KSX - 1; ++ks) a[rt][ks] = frag[((size_t)(t0 + rt) * KSX + ks) * 64 + lane];
+        a[rt][KSX - 1] = frag_a8[(size_t)(t0 + rt) * 64 + lane];
     }
     // accumulator r of row tile rt holds row row0 + 32*rt + rbase + r%4 + 8*(r/4)
     const uint32_t rbase = 4 * (lane >> 5);
     const uint32_t shard = wave_shard();
+    const uint64_t below = (1ull << lane) - 1ull;
 
     int buf = 0;
     for (uint32_t c0 = c_begin; c0 < c_end; c0 += M_STEP, buf ^= 1) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (c0 + M_STEP < c_end) issue_btile(frag, c0 + M_STEP, Bl[buf ^ 1], tid, wave);
+        if (c0 + M_STEP < c_end) issue_btile<KSX>(frag, c0 + M_STEP, Bl[buf ^ 1], tid, wave);
         const uint4* B = Bl[buf];
         // pairs with j <= i exist only where the step meets the diagonal
         const bool diag = c0 < row0 + 64;
@@ -548,57 +626,52 @@ __global__ __launch_bounds__(256, 2) void k_prefilter_mfma(const uint4* __restri
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = f32x16_t{};
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
+            for (int ks = 0; ks < KSX; ++ks) {
                 uint4 b[2];
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct) b[ct] = B[((2 * hc + ct) * KS + ks) * 64 + lane];
+                for (int ct = 0; ct < 2; ++ct) b[ct] = B[((2 * hc + ct) * KSX + ks) * 64 + lane];
 #pragma unroll
                 for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                     for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma32<BF16>(a[rt][ks], b[ct], acc[rt][ct]);
             }
-            const uint32_t jl = c0 + 64 * hc + (lane & 31);
-            uint32_t tcnt[2][2];
-            uint32_t total = 0;
+            // fast path: per-lane maxima of each 32x32 tile
+            float tmax[2][2];
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
-                    uint32_t c = 0;
+                    const f32x16_t v = acc[rt][ct];
+                    float m0 = fmaxf(fmaxf(v[0], v[1]), v[2]), m1 = fmaxf(fmaxf(v[3], v[4]), v[5]);
+                    float m2 = fmaxf(fmaxf(v[6], v[7]), v[8]), m3 = fmaxf(fmaxf(v[9], v[10]), v[11]);
+                    float m4 = fmaxf(fmaxf(v[12], v[13]), v[14]);
+                    tmax[rt][ct] = fmaxf(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)), fmaxf(m4, v[15]));
+                }
+            const float hmax = fmaxf(fmaxf(tmax[0][0], tmax[0][1]), fmaxf(tmax[1][0], tmax[1][1]));
+            if (!__any(hmax > 0.f)) continue;
+            const uint32_t jl = c0 + 64 * hc + (lane & 31);
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    if (!__any(tmax[rt][ct] > 0.f)) continue;
+                    const uint32_t j = jl + 32 * ct;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const uint32_t i = row0 + 32 * rt + rbase + (r & 3) + 8 * (r >> 2);
-                        const bool h = acc[rt][ct][r] > 0.f && (!diag || i < jl + 32 * ct);
-                        c += (uint32_t)__popcll(__ballot(h));
-                    }
-                    tcnt[rt][ct] = c;
-                    total += c;
-                }
-            if (total) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&out.cnt[(size_t)shard * SHARD_STRIDE], (unsigned long long)total);
-                base = __shfl(base, 0, 64);
-                const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-                for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-                    for (int ct = 0; ct < 2; ++ct) {
-                        if (tcnt[rt][ct] == 0) continue;
-#pragma unroll
-                        for (int r = 0; r < 16; ++r) {
-                            const uint32_t i = row0 + 32 * rt + rbase + (r & 3) + 8 * (r >> 2);
-                            const uint32_t j = jl + 32 * ct;
-                            const bool h = acc[rt][ct][r] > 0.f && (!diag || i < j);
-                            const uint64_t m = __ballot(h);
-                            if (m) {
-                                if (h) shard_put(out, shard, base + (uint64_t)__popcll(m & below), i, j);
-                                base += (unsigned long long)__popcll(m);
-                            }
+                        const bool h = acc[rt][ct][r] > 0.f && (!diag || i < j);
+                        const uint64_t m = __ballot(h);
+                        if (m) {
+                            const uint32_t c = (uint32_t)__popcll(m);
+                            if (nstage + c > M_STAGE) stage_flush(out, shard, stage, nstage, lane);
+                            if (h) stage[nstage + (uint32_t)__popcll(m & below)] = make_uint2(i, j);
+                            nstage += c;
                         }
                     }
-            }
+                }
         }
     }
+    stage_flush(out, shard, stage, nstage, lane);
 }
 
 // ---------------------------------------------------------------- refine --
@@ -607,6 +680,7 @@ __global__ __launch_bounds__(256, 2) void k_prefilter_mfma(const uint4* __restri
 // evaluates the model.  16 survivors per wave in flight, 64-byte contiguous
 // segments per row per load instruction.
 __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, uint64_t ns,
+                                                const uint32_t* __restrict__ perm,
                                                 const float* __restrict__ tnf, const double* __restrict__ ls,
                                                 const float* __restrict__ tvec, double T, Cand* __restrict__ cand,
                                                 unsigned long long* __restrict__ counters) {
@@ -624,6 +698,7 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
         float part = 0.f;
         if (idx < ns) {
             p = surv[idx];
+            if (perm) p = make_uint2(perm[p.x], perm[p.y]);  // sorted positions -> contig ids
             const float4* a4 = reinterpret_cast<const float4*>(tnf + (size_t)p.x * NT);
             const float4* b4 = reinterpret_cast<const float4*>(tnf + (size_t)p.y * NT);
             float s0 = 0.f, s1 = 0.f;
@@ -871,6 +946,126 @@ __global__ void k_emit(const uint64_t* __restrict__ rowptr, const Adj* __restric
     }
 }
 
+// ------------------------------------------------------------- row max --
+// Sampled row maxima of preProb (get_tnf_max_prob_sample, cuda/Cuda2.cu:
+// 281-314): for each sampled position p < n_sample, the maximum over all
+// positions q != p of preProb(idx[p], idx[q]).  Block = 64 sampled rows x a
+// chunk of columns; both sides staged K-major in LDS, each thread owns a 4x4
+// block of pairs: FP32 squared distance, FP32 model, running row maximum.
+// Pass 1 (EXACT=false) finds the FP32 maximum m32 of each row; pass 2
+// (EXACT=true) re-evaluates in FP64 every pair with FP32 preProb >= m32 - 2*EPS
+// (the true maximum is among them) and keeps their FP64 maximum.
+constexpr int RM_ROWS = 64, RM_COLS = 64, RM_CHUNK = 4096;
+
+__device__ inline uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ord2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+__device__ inline unsigned long long d2ord(double f) {
+    unsigned long long u = (unsigned long long)__double_as_longlong(f);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double ord2d(unsigned long long u) {
+    return __longlong_as_double((long long)((u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u));
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void k_rowmax(const float* __restrict__ tnf, const double* __restrict__ ls,
+                                                const float* __restrict__ tvec, const uint64_t* __restrict__ idx,
+                                                uint32_t n, uint32_t n_sample, uint32_t* __restrict__ m32,
+                                                unsigned long long* __restrict__ m64) {
+    __shared__ float As[NT][RM_ROWS + 1];
+    __shared__ float Bs[NT][RM_COLS + 1];
+    __shared__ uint32_t rid[RM_ROWS], cid[RM_COLS];
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    const uint32_t p0 = blockIdx.x * RM_ROWS;
+    const uint32_t q_begin = blockIdx.y * RM_CHUNK, q_end = min(q_begin + RM_CHUNK, n);
+    if (tid < RM_ROWS) rid[tid] = (p0 + tid < n_sample) ? (uint32_t)idx[p0 + tid] : 0u;
+    __syncthreads();
+    for (int e = tid; e < RM_ROWS * NT; e += 256) {
+        const int r = e / NT, k = e - r * NT;
+        As[k][r] = tnf[(size_t)rid[r] * NT + k];
+    }
+    float best[4];
+    float thr[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        best[a] = -__builtin_inff();
+        const uint32_t p = p0 + ty + 16 * a;
+        thr[a] = (EXACT && p < n_sample) ? ord2f(m32[p]) - 2.f * EPS_PRE : 0.f;
+    }
+    double best64[4] = {-1.0 / 0.0, -1.0 / 0.0, -1.0 / 0.0, -1.0 / 0.0};
+    for (uint32_t q0 = q_begin; q0 < q_end; q0 += RM_COLS) {
+        __syncthreads();
+        if (tid < RM_COLS) cid[tid] = (q0 + tid < q_end) ? (uint32_t)idx[q0 + tid] : 0u;
+        __syncthreads();
+        for (int e = tid; e < RM_COLS * NT; e += 256) {
+            const int c = e / NT, k = e - c * NT;
+            Bs[k][c] = tnf[(size_t)cid[c] * NT + k];
+        }
+        __syncthreads();
+        float acc[4][4] = {};
+        for (int k = 0; k < NT; ++k) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[k][ty + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[k][tx + 16 * b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const float df = av[a] - bv[b];
+                    acc[a][b] = fmaf(df, df, acc[a][b]);
+                }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const uint32_t p = p0 + ty + 16 * a;
+            if (p >= n_sample) continue;
+            const uint32_t ri = rid[ty + 16 * a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t q = q0 + tx + 16 * b;
+                if (q >= q_end || q == p) continue;
+                const uint32_t cj = cid[tx + 16 * b];
+                const float pre = model32(sqrtf(acc[a][b]), tvec[ri], tvec[cj]).pre;
+                if (!EXACT) {
+                    best[a] = fmaxf(best[a], pre);
+                } else if (pre >= thr[a]) {
+                    const double v = model64(tnf + (size_t)ri * NT, tnf + (size_t)cj * NT, ls[ri], ls[cj]);
+                    best64[a] = v > best64[a] ? v : best64[a];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const uint32_t p = p0 + ty + 16 * a;
+        if (p >= n_sample) continue;
+        if (!EXACT) {
+            if (best[a] > -__builtin_inff()) atomicMax(&m32[p], f2ord(best[a]));
+        } else if (best64[a] > -1.0 / 0.0) {
+            atomicMax(&m64[p], d2ord(best64[a]));
+        }
+    }
+}
+
+// max_out[p] = 1 - 1/(1 + exp(max(DBL_MIN, m64[p]))): the running maximum of
+// the reference starts at DBL_MIN (cuda/Cuda2.cu:287, 312).
+__global__ void k_rowmax_final(const unsigned long long* __restrict__ m64, uint32_t n_sample,
+                               double* __restrict__ out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_sample) return;
+    double m = 2.2250738585072014e-308;
+    if (m64[p]) {
+        const double v = ord2d(m64[p]);
+        if (v > m) m = v;
+    }
+    out[p] = 1. - (1. / (1. + exp(m)));
+}
+
 __global__ void k_preprob_pairs(const float* __restrict__ tnf, const double* __restrict__ ls,
                                 const float* __restrict__ tvec, const uint32_t* __restrict__ pi,
                                 const uint32_t* __restrict__ pj, uint64_t np, float* __restrict__ p32,
@@ -939,6 +1134,25 @@ int exclusive_scan_u32_to_u64(mdet_ctx* ctx, const uint32_t* in, uint64_t* out, 
     return MDET_OK;
 }
 
+// Contig order of the MFMA prefilter (ascending composition key).
+int sort_by_composition(mdet_ctx* ctx, const float* tnf_dev, uint32_t n, uint32_t** perm_out) {
+    float *key, *key_sorted;
+    uint32_t *idx, *perm;
+    MDET_TRY(ctx_buf(ctx, "sort.key", n, &key));
+    MDET_TRY(ctx_buf(ctx, "sort.key2", n, &key_sorted));
+    MDET_TRY(ctx_buf(ctx, "sort.idx", n, &idx));
+    MDET_TRY(ctx_buf(ctx, "sort.perm", n, &perm));
+    hipLaunchKernelGGL(k_sort_key, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, tnf_dev, n, key, idx);
+    MDET_HIP(hipGetLastError());
+    size_t tmp = 0;
+    MDET_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key_sorted, idx, perm, (int)n, 0, 32, ctx->stream));
+    void* t;
+    MDET_TRY(ctx_buf(ctx, "cub.sorttmp", tmp, &t));
+    MDET_HIP(hipcub::DeviceRadixSort::SortPairs(t, tmp, key, key_sorted, idx, perm, (int)n, 0, 32, ctx->stream));
+    *perm_out = perm;
+    return MDET_OK;
+}
+
 // Prefilter items for the MFMA kernel: (row block, column begin, column end).
 int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nitems) {
     if (ctx->items_npad == n_pad && ctx->nitems) {
@@ -968,16 +1182,24 @@ int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nite
 
 extern "C" {
 
-int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
-                   uint32_t max_edges, int precision, mdet_edges* out, mdet_pair_stats* stats) {
+}  // extern "C"
+
+namespace {
+
+int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
+               uint32_t max_edges, int precision, mdet_mlp* mlp, float radius2, mdet_edges* out,
+               mdet_pair_stats* stats) {
     MDET_TRY(check_ctx(ctx));
     if (!out) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: null output");
+    ctx->last_pairs = 0;
     *out = mdet_edges{0, nullptr, nullptr, nullptr};
     mdet_pair_stats st{};
     st.pairs_total = (uint64_t)n * (n ? n - 1 : 0) / 2;
     if (!(cutoff > 0.0 && cutoff < 1.0)) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: cutoff %g not in (0, 1)", cutoff);
-    if (precision < MDET_EXACT_F32 || precision > MDET_F16_REFINE)
+    if (precision < MDET_EXACT_F32 || precision > MDET_LEARNED)
         MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: unknown precision %d", precision);
+    if (precision == MDET_LEARNED && (!mlp || !(radius2 > 0)))
+        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges_learned: needs a network and a positive radius");
     if (n > (1u << 30)) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: n = %u too large", n);
     if (n < 2) {
         if (stats) *stats = st;
@@ -1047,6 +1269,7 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
         if (it_cap != ctx->bufs.end())
             shard_cap = std::max<uint64_t>(shard_cap, it_cap->second.bytes / sizeof(uint2) / NSHARD);
         uint2 *sbuf = nullptr, *surv = nullptr;
+        uint32_t* perm = nullptr;  // sorted position -> contig id (MFMA modes)
         uint64_t ns = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
             MDET_TRY(ctx_buf(ctx, "pairs.sbuf", (size_t)NSHARD * shard_cap, &sbuf));
@@ -1064,6 +1287,23 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
                 const float R2 = (float)(R * R * (1.0 + 1e-4)) + 1e-6f;
                 hipLaunchKernelGGL(k_prefilter_f32, dim3((unsigned)ntri), dim3(256), 0, ctx->stream, tnft, n, n_pad,
                                    ntiles, R2, so);
+            } else if (precision == MDET_LEARNED) {
+                // embedding rows (n x 32) in BF16, threshold = calibrated radius^2
+                float* emb;
+                MDET_TRY(ctx_buf(ctx, "learn.emb", (size_t)n * 32, &emb));
+                if (attempt == 0) MDET_TRY(mdet_mlp_embed(mlp, tnf_dev, n, emb));
+                uint4 *frag, *frag_a;
+                MDET_TRY(ctx_buf(ctx, "learn.frag", (size_t)n_pad / 32 * KS_E * 64, &frag));
+                MDET_TRY(ctx_buf(ctx, "learn.frag_a", (size_t)n_pad / 32 * 64, &frag_a));
+                MDET_TRY(sort_by_composition(ctx, tnf_dev, n, &perm));
+                hipLaunchKernelGGL((k_prep_lowp<true, 32, KS_E>), dim3(n_pad / 4), dim3(256), 0, ctx->stream, emb,
+                                   perm, n, n_pad, radius2, frag, frag_a);
+                uint4* items;
+                uint32_t nitems;
+                MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                if (attempt == 0) MDET_TRY(tm.record(1));
+                hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), dim3(nitems), dim3(256), 0, ctx->stream, frag, frag_a,
+                                   items, n, so);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
                 // |d~ - d| <= |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit
@@ -1074,21 +1314,22 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
                 uint4 *frag, *frag_a8;
                 MDET_TRY(ctx_buf(ctx, "pairs.frag", (size_t)n_pad / 32 * KS * 64, &frag));
                 MDET_TRY(ctx_buf(ctx, "pairs.frag_a8", (size_t)n_pad / 32 * 64, &frag_a8));
+                MDET_TRY(sort_by_composition(ctx, tnf_dev, n, &perm));
                 if (bf)
-                    hipLaunchKernelGGL(k_prep_lowp<true>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
-                                       n_pad, R2, frag, frag_a8);
+                    hipLaunchKernelGGL((k_prep_lowp<true, NT, KS>), dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev,
+                                       perm, n, n_pad, R2, frag, frag_a8);
                 else
-                    hipLaunchKernelGGL(k_prep_lowp<false>, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, n,
-                                       n_pad, R2, frag, frag_a8);
+                    hipLaunchKernelGGL((k_prep_lowp<false, NT, KS>), dim3(n_pad / 4), dim3(256), 0, ctx->stream,
+                                       tnf_dev, perm, n, n_pad, R2, frag, frag_a8);
                 uint4* items;
                 uint32_t nitems;
                 MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (bf)
-                    hipLaunchKernelGGL(k_prefilter_mfma<true>, dim3(nitems), dim3(256), 0, ctx->stream, frag, frag_a8,
-                                       items, n, so);
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
+                                       frag_a8, items, n, so);
                 else
-                    hipLaunchKernelGGL(k_prefilter_mfma<false>, dim3(nitems), dim3(256), 0, ctx->stream, frag,
+                    hipLaunchKernelGGL((k_prefilter_mfma<false, KS>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
                                        frag_a8, items, n, so);
             }
             MDET_HIP(hipGetLastError());
@@ -1122,7 +1363,7 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
         MDET_TRY(ctx_buf(ctx, "pairs.cand", std::max<uint64_t>(ns, 1), &cand));
         if (ns) {
             const uint64_t blocks = std::min<uint64_t>((ns + 63) / 64, (uint64_t)ctx->num_cus * 32);
-            hipLaunchKernelGGL(k_refine, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, surv, ns, tnf_dev,
+            hipLaunchKernelGGL(k_refine, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, surv, ns, perm, tnf_dev,
                                logsize_dev, tvec, T, cand, rc);
             MDET_HIP(hipGetLastError());
         }
@@ -1136,6 +1377,7 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
             memcpy(rch, st_host, sizeof rch);
         }
         const uint64_t nc = ns;  // candidate records (kept ones flagged)
+        ctx->last_pairs = ns;
         st.candidates = rch[0];
         st.adjudicated = rch[1];
 
@@ -1204,6 +1446,72 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
     st.ms_total = tm.ms(0, 4);
     *out = result;
     if (stats) *stats = st;
+    return MDET_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
+                   uint32_t max_edges, int precision, mdet_edges* out, mdet_pair_stats* stats) {
+    if (precision == MDET_LEARNED)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: the learned mode needs mdet_tnf_edges_learned");
+    return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, precision, nullptr, 0.f, out, stats);
+}
+
+int mdet_tnf_edges_learned(mdet_ctx* ctx, mdet_mlp* mlp, const float* tnf_dev, const double* logsize_dev, uint32_t n,
+                           double cutoff, uint32_t max_edges, float radius2, mdet_edges* out, mdet_pair_stats* stats) {
+    return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, MDET_LEARNED, mlp, radius2, out, stats);
+}
+
+int mdet_last_pairs(mdet_ctx* ctx, const void** records_dev, uint64_t* count) {
+    MDET_TRY(check_ctx(ctx));
+    if (!records_dev || !count) MDET_FAIL(MDET_E_INVALID, "mdet_last_pairs: null output");
+    auto it = ctx->bufs.find("pairs.cand");
+    *records_dev = it == ctx->bufs.end() ? nullptr : it->second.p;
+    *count = it == ctx->bufs.end() ? 0 : ctx->last_pairs;
+    return MDET_OK;
+}
+
+int mdet_tnf_rowmax(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, const uint64_t* idx_dev,
+                    uint32_t n_sample, double* max_out_dev) {
+    MDET_TRY(check_ctx(ctx));
+    if (n_sample == 0) return MDET_OK;
+    if (!tnf_dev || !logsize_dev || !idx_dev || !max_out_dev) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_rowmax: null argument");
+    if (n_sample > n) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_rowmax: n_sample %u > n %u", n_sample, n);
+    MDET_TRY(ensure_model(ctx));
+    float* tvec;
+    unsigned long long* mm;
+    uint32_t* m32;
+    unsigned long long* m64;
+    MDET_TRY(ctx_buf(ctx, "rm.tvec", n, &tvec));
+    MDET_TRY(ctx_buf(ctx, "rm.mm", 8, &mm));
+    MDET_TRY(ctx_buf(ctx, "rm.m32", n_sample, &m32));
+    MDET_TRY(ctx_buf(ctx, "rm.m64", n_sample, &m64));
+    {
+        unsigned long long init[4] = {~0ull, 0ull, 0ull, 0ull};
+        void* st_host;
+        MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
+        memcpy(st_host, init, sizeof init);
+        MDET_HIP(hipMemcpyAsync(mm, st_host, sizeof init, hipMemcpyHostToDevice, ctx->stream));
+    }
+    hipLaunchKernelGGL(k_prep_rows, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, logsize_dev, n, tvec, mm,
+                       reinterpret_cast<unsigned int*>(mm + 2));
+    unsigned long long bad = 0;
+    MDET_TRY(read_scalar(ctx, mm + 2, &bad));
+    if ((unsigned int)bad != 0)
+        MDET_FAIL(MDET_E_RANGE, "mdet_tnf_rowmax: %u log sizes outside [log10 1500, log10 500000]", (unsigned int)bad);
+    MDET_HIP(hipMemsetAsync(m32, 0, (size_t)n_sample * 4, ctx->stream));
+    MDET_HIP(hipMemsetAsync(m64, 0, (size_t)n_sample * 8, ctx->stream));
+    const dim3 grid((n_sample + RM_ROWS - 1) / RM_ROWS, (n + RM_CHUNK - 1) / RM_CHUNK);
+    hipLaunchKernelGGL(k_rowmax<false>, grid, dim3(256), 0, ctx->stream, tnf_dev, logsize_dev, tvec, idx_dev, n,
+                       n_sample, m32, m64);
+    hipLaunchKernelGGL(k_rowmax<true>, grid, dim3(256), 0, ctx->stream, tnf_dev, logsize_dev, tvec, idx_dev, n,
+                       n_sample, m32, m64);
+    hipLaunchKernelGGL(k_rowmax_final, dim3((n_sample + 255) / 256), dim3(256), 0, ctx->stream, m64, n_sample,
+                       max_out_dev);
+    MDET_HIP(hipGetLastError());
     return MDET_OK;
 }
 

@@ -1,0 +1,127 @@
+"""GPU tests of the sampled row maximum (automatic pTNF) and of the C++
+drop-in program, against the oracle."""
+import ctypes
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import mdet_amd as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "mdet_amd", "bin", "mdet_tnf_graph")
+LIBC = ctypes.CDLL("libc.so.6")
+
+
+def _contigs(n, seed, lmin=1500, lmax=80000, genomes=8):
+    cfg = M.gen_config(seed=seed, n_genomes=genomes, len_median=5000, len_sigma=1.0, len_min=lmin, len_max=lmax)
+    lens, _ = M.gen_lengths(cfg, n)
+    return [M.gen_ascii(cfg, i, int(lens[i])) for i in range(n)]
+
+
+def test_rowmax_matches_oracle(ctx):
+    seqs = _contigs(900, 3)
+    tnf = O.normalize_f32(np.stack([O.count_seq(s) for s in seqs]))
+    ls = O.logsize([len(s) for s in seqs])
+    n = len(seqs)
+    idx = np.random.default_rng(4).permutation(n).astype(np.uint64)
+    dt, dl = ctx.to_device(tnf), ctx.to_device(ls)
+    got = ctx.tnf_rowmax(dt, dl, n, idx, 300)
+    want = O.rowmax(tnf, ls, idx, 300)
+    print(f"rowmax: max |diff| {np.max(np.abs(got - want)):.3e}", flush=True)
+    assert np.max(np.abs(got - want)) <= 4e-16  # same preProb maximum; exp() may differ in the last bit
+    dt.free()
+    dl.free()
+
+
+def _write_fasta(path, seqs):
+    with open(path, "wb") as f:
+        for i, s in enumerate(seqs):
+            f.write(f">c{i} x\n".encode())
+            for p in range(0, len(s), 80):
+                f.write(s[p:p + 80] + b"\n")
+
+
+def _run_cli(args):
+    r = subprocess.run([CLI] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stderr
+
+
+def _read_edges(path):
+    out = set()
+    for line in open(path):
+        a, b, s = line.split("\t")
+        out.add((a, b))
+    return out
+
+
+def _oracle_named_edges(seqs, min_contig, cutoff, K):
+    big = [i for i, s in enumerate(seqs) if len(s) >= min_contig]
+    tnf = O.normalize_f32(np.stack([O.count_seq(seqs[i]) for i in big]))
+    ls = O.logsize([len(seqs[i]) for i in big])
+    f, t, _ = O.edges(tnf, ls, cutoff, K)
+    return {(f"c{big[a]}", f"c{big[b]}") for a, b in zip(f.tolist(), t.tolist())}, tnf, ls
+
+
+def test_cli_fixed_cutoff_matches_oracle(tmp_path):
+    seqs = _contigs(700, 5, lmin=1200)
+    fa = tmp_path / "c.fa"
+    _write_fasta(fa, seqs)
+    out = tmp_path / "e.tsv"
+    err = _run_cli(["-i", str(fa), "-o", str(out), "-m", "2500", "--pTNF", "95", "--maxEdges", "30"])
+    print(err, flush=True)
+    want, _, _ = _oracle_named_edges(seqs, 2500, 0.95, 30)
+    assert _read_edges(out) == want and len(want) > 50
+
+
+def _expected_auto_ptnf(tnf, ls, seed, maxP):
+    """main()'s pTNF for nobs <= 25000 (cuda/Cuda2.cu:1886-1887, 972-1025) with
+    glibc rand(), restated from the reference over the oracle's row maxima."""
+    LIBC.srand(seed)
+    n = tnf.shape[0]
+    idx = list(range(n))
+    left = n
+    for k in range(n):  # full sample
+        r = k + LIBC.rand() % left
+        idx[k], idx[r] = idx[r], idx[k]
+        left -= 1
+    mx = sorted(O.rowmax(tnf, ls, np.array(idx, np.uint64), n).tolist(), reverse=True)
+    p, pp, cov, pcov, counton = 999, 1000, 0.0, 0.0, 0
+    while p > 700:
+        cutoff = p / 1000.0
+        while counton < n and mx[counton] >= cutoff:
+            counton += 1
+        cov = counton / n
+        if cov >= maxP:
+            if cov - maxP > maxP - pcov:
+                p, cov = pp, pcov
+            break
+        pp, pcov = p, cov
+        if p > 990:
+            p -= LIBC.rand() % 3 + 1
+        elif p > 900:
+            p -= LIBC.rand() % 3 + 3
+        else:
+            p -= LIBC.rand() % 3 + 9
+    return p
+
+
+def test_cli_automatic_ptnf_matches_oracle(tmp_path):
+    seqs = _contigs(500, 9)
+    fa = tmp_path / "c.fa"
+    _write_fasta(fa, seqs)
+    out = tmp_path / "e.tsv"
+    err = _run_cli(["-i", str(fa), "-o", str(out), "-m", "2500", "--pTNF", "0", "--seed", "7", "--maxEdges", "200"])
+    print(err, flush=True)
+    got = float(re.search(r"pTNF = ([0-9.]+)", err).group(1))
+    _, tnf, ls = _oracle_named_edges(seqs, 2500, 0.9, 200)
+    p = _expected_auto_ptnf(tnf, ls, 7, 0.95)
+    assert abs(got - p / 10.0) < 1e-9, (got, p)
+    want, _, _ = _oracle_named_edges(seqs, 2500, p / 1000.0, 200)
+    assert _read_edges(out) == want

@@ -252,3 +252,31 @@ def test_oracle_rowmax_bruteforce():
         m = max([O.preprob(tnf[idx[p]], tnf[idx[q]], ls[idx[p]], ls[idx[q]]) for q in range(n) if q != p]
                 + [2.2250738585072014e-308])
         assert got[p] == 1 - 1 / (1 + math.exp(m))
+
+
+# ------------------------------------------------------ learned-filter MLP --
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(23)
+    p = rng.normal(0, 0.1, MO.N_PARAMS)
+    tnf = rng.dirichlet(np.ones(136), size=12).astype(np.float32)
+    mean, istd = tnf.mean(0), 1.0 / (tnf.std(0) + 1e-3)
+    pi, pj = np.arange(0, 6), np.arange(6, 12)
+    y = np.array([1, 0, 1, 0, 0, 1], np.float64)
+    L, g = MO.loss_and_grad(p, mean, istd, tnf, pi, pj, y, margin=4.0, round_weights=False)
+    assert L > 0
+    for k in list(rng.choice(MO.N_PARAMS, 25, replace=False)) + [MO.N_PARAMS - 1, 0]:
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        num = (MO.loss_and_grad(pp, mean, istd, tnf, pi, pj, y, 4.0, False)[0] -
+               MO.loss_and_grad(pm, mean, istd, tnf, pi, pj, y, 4.0, False)[0]) / (2 * h)
+        assert abs(num - g[k]) <= 1e-6 + 1e-5 * abs(num), (k, num, g[k])
+
+
+def test_mlp_oracle_bf16_rounding_is_round_to_nearest_even():
+    from oracle import mlp_oracle as MO
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e-40], np.float32)
+    got = MO.bf16_round(v)
+    assert got[0] == 1.0 and got[1] == 1.0 and got[2] == 1.0 + 2 ** -6 and got[3] == -2.5
