@@ -205,7 +205,9 @@ int mdet_mlp_create(mdet_ctx *ctx, uint64_t seed, mdet_mlp **out);
 int mdet_mlp_destroy(mdet_mlp *mlp);
 uint64_t mdet_mlp_param_count(void); /* 109,088 */
 /* FP32 master parameters, layout W1[256x136] b1[256] W2[256x256] b2[256]
- * W3[32x256] b3[32] (row-major, out x in); host arrays. */
+ * W3[32x256] b3[32] (row-major, out x in); host arrays.  If count is at least
+ * twice the parameter count, the second half receives the parameters as the
+ * forward pass uses them (weights rounded to BF16). */
 int mdet_mlp_get_params(mdet_mlp *mlp, float *out, uint64_t count);
 int mdet_mlp_set_params(mdet_mlp *mlp, const float *in, uint64_t count);
 /* Input standardisation x' = (tnf - mean) * istd, per column (host arrays). */

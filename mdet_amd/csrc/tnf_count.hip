@@ -100,30 +100,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
     int64_t c_last = s_c[1] > (int64_t)n - 1 ? (int64_t)n - 1 : s_c[1];
     const int64_t r_first = s_r[0], r_last = s_r[1];
 
-    // Per-word data is loaded once and kept for all passes.
-    uint64_t xw[CNT_WORDS];
-    uint32_t nmask[CNT_WORDS];  // bit j: base (16w - 3 + j) is N, j < 19
     const uint64_t w0 = chunk_lo / 16;
-#pragma unroll
-    for (int it = 0; it < CNT_WORDS; ++it) {
-        const uint64_t w = w0 + tid + (uint64_t)it * CNT_THREADS;
-        const uint64_t p0 = w * 16;
-        xw[it] = 0;
-        nmask[it] = 0;
-        if (p0 < chunk_hi) {
-            uint32_t cur = packed[w];
-            uint32_t prev = w ? packed[w - 1] : 0u;
-            xw[it] = ((uint64_t)cur << 32) | prev;
-            const int64_t span_lo = (int64_t)p0 - 3, span_hi = (int64_t)p0 + 16;
-            for (int64_t r = r_first; r < r_last; ++r) {
-                int64_t rs = (int64_t)runs[2 * r], re = (int64_t)runs[2 * r + 1];
-                int64_t lo = rs > span_lo ? rs : span_lo;
-                int64_t hi = re < span_hi ? re : span_hi;
-                if (lo < hi) nmask[it] |= ((1u << (uint32_t)(hi - lo)) - 1u) << (uint32_t)(lo - span_lo);
-            }
-        }
-    }
-
     for (int64_t cs = c_first; cs <= c_last; cs += CNT_MAXSEG) {
         const int nseg = (int)min<int64_t>(CNT_MAXSEG, c_last - cs + 1);
         if (tid <= nseg) seg_lo[tid] = base_off[cs + tid];
@@ -131,24 +108,45 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
         for (int i = tid; i < CNT_MAXSEG * 136; i += CNT_THREADS) (&fold[0][0])[i] = 0;
         __syncthreads();
         const uint64_t pass_lo = seg_lo[0], pass_hi = seg_lo[nseg];
-#pragma unroll
+#pragma unroll 1
         for (int it = 0; it < CNT_WORDS; ++it) {
-            const uint64_t p0 = (w0 + tid + (uint64_t)it * CNT_THREADS) * 16;
+            const uint64_t w = w0 + tid + (uint64_t)it * CNT_THREADS;
+            const uint64_t p0 = w * 16;
             if (p0 >= chunk_hi || p0 + 16 <= pass_lo || p0 >= pass_hi) continue;
-            // segment of the word's first position, then walk forward
+            const uint32_t cur = packed[w];
+            const uint32_t prev = w ? packed[w - 1] : 0u;
+            const uint64_t x = ((uint64_t)cur << 32) | prev;
+            // bit j of nm: base (p0 - 3 + j) is N, j < 19
+            uint32_t nm = 0;
+            const int64_t span_lo = (int64_t)p0 - 3, span_hi = (int64_t)p0 + 16;
+            for (int64_t r = r_first; r < r_last; ++r) {
+                const int64_t rs = (int64_t)runs[2 * r], re = (int64_t)runs[2 * r + 1];
+                const int64_t lo = rs > span_lo ? rs : span_lo;
+                const int64_t hi = re < span_hi ? re : span_hi;
+                if (lo < hi) nm |= ((1u << (uint32_t)(hi - lo)) - 1u) << (uint32_t)(lo - span_lo);
+            }
+            // window k (ending at p0 + k) is N-free iff bits k..k+3 of nm are clear
+            const uint32_t bad = nm | (nm >> 1) | (nm >> 2) | (nm >> 3);
             int s = 0;
             while (s + 1 <= nseg && seg_lo[s + 1] <= p0) ++s;
-            const uint64_t x = xw[it];
-            const uint32_t nm = nmask[it];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const uint64_t p = p0 + k;
-                while (s < nseg && seg_lo[s + 1] <= p) ++s;
-                bool ok = (p >= chunk_lo) && (p < chunk_hi) && (p >= pass_lo) && (s < nseg) &&
-                          (p >= seg_lo[s] + 3) && (((nm >> k) & 0xFu) == 0);
-                if (ok) {
-                    uint32_t code = (uint32_t)(x >> (26 + 2 * k)) & 0xFFu;
-                    atomicAdd(&hist[s][code], 1u);
+            const uint64_t seg_end = (s < nseg) ? seg_lo[s + 1] : 0;
+            if (s < nseg && p0 >= seg_lo[s] + 3 && p0 + 16 <= seg_end && p0 >= chunk_lo && p0 + 16 <= chunk_hi) {
+                // the whole word belongs to contig cs + s: 16 windows, N mask only
+                uint32_t valid = ~bad & 0xFFFFu;
+                uint32_t* h = hist[s];
+                while (valid) {
+                    const int k = __builtin_ctz(valid);
+                    valid &= valid - 1;
+                    atomicAdd(&h[(uint32_t)(x >> (26 + 2 * k)) & 0xFFu], 1u);
+                }
+            } else {
+#pragma unroll 1
+                for (int k = 0; k < 16; ++k) {
+                    const uint64_t p = p0 + k;
+                    while (s < nseg && seg_lo[s + 1] <= p) ++s;
+                    const bool ok = (p >= chunk_lo) && (p < chunk_hi) && (p >= pass_lo) && (s < nseg) &&
+                                    (p >= seg_lo[s] + 3) && (((bad >> k) & 1u) == 0);
+                    if (ok) atomicAdd(&hist[s][(uint32_t)(x >> (26 + 2 * k)) & 0xFFu], 1u);
                 }
             }
         }

@@ -118,6 +118,10 @@ int main(int argc, char** argv) {
         std::ofstream o(tnf_out, std::ios::binary);
         o.write(reinterpret_cast<const char*>(h.data()), h.size() * sizeof(float));
     }
+    // Seed the C RNG right before the sample: the HIP runtime may draw from
+    // it while the device initialises, and the sample must see the sequence
+    // srand(seed) starts (cuda/Cuda2.cu:1311-1312 seeds before any rand()).
+    srand((unsigned)seed);
     const double ptnf = path.select_ptnf(pTNF, maxP / 100.);
     double t3 = now();
     mdet::Graph g(path.nobs());

@@ -172,11 +172,17 @@ __global__ void k_adam(float* __restrict__ p, float* __restrict__ wq, const floa
     wq[k] = is_bias ? w : bf16_round(w);
 }
 
-__global__ void k_quantize(const float* __restrict__ p, float* __restrict__ wq, uint32_t count) {
+// One parameter block [off, off + count): weights are rounded to BF16, biases
+// copied.  The block kind is a launch argument, uniform over the grid: a
+// per-lane bias test here (branching between a copy and the rounding) was
+// compiled by this hipcc into a store whose address register the range compare
+// had overwritten, so all-bias waves wrote one element each.
+__global__ void k_quantize(const float* __restrict__ p, float* __restrict__ wq, uint32_t off, uint32_t count,
+                           int round) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
-    const bool is_bias = (k >= OFF_B1 && k < OFF_W2) || (k >= OFF_B2 && k < OFF_W3) || k >= OFF_B3;
-    wq[k] = is_bias ? p[k] : bf16_round(p[k]);
+    const float v = p[off + k];
+    wq[off + k] = round ? bf16_round(v) : v;
 }
 
 }  // namespace
@@ -197,6 +203,27 @@ struct mdet_mlp {
 namespace {
 
 using namespace mdet;
+
+// Host<->device transfers go through the context's stream and complete before
+// returning: that stream is non-blocking, so a null-stream hipMemcpy would not
+// be ordered before the kernels launched on it afterwards.
+int mlp_copy(mdet_mlp* m, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    MDET_HIP(hipMemcpyAsync(dst, src, bytes, kind, m->ctx->stream));
+    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
+    return MDET_OK;
+}
+
+// wq = p with the weights rounded to BF16, block by block
+int mlp_quantize(mdet_mlp* m) {
+    static const uint32_t blocks[6][3] = {{OFF_W1, N_W1, 1}, {OFF_B1, D_H, 0}, {OFF_W2, N_W2, 1},
+                                          {OFF_B2, D_H, 0}, {OFF_W3, N_W3, 1}, {OFF_B3, D_E, 0}};
+    for (const auto& b : blocks)
+        hipLaunchKernelGGL(k_quantize, dim3((b[1] + 255) / 256), dim3(256), 0, m->ctx->stream, m->p, m->wq, b[0], b[1],
+                           (int)b[2]);
+    MDET_HIP(hipGetLastError());
+    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
+    return MDET_OK;
+}
 
 int mlp_alloc(mdet_mlp* m, size_t bytes, void** out) {
     hipError_t e = hipMalloc(out, bytes ? bytes : 4);
@@ -322,13 +349,17 @@ int mdet_mlp_create(mdet_ctx* ctx, uint64_t seed, mdet_mlp** out) {
     init_params(p, seed);
     std::vector<float> norm(2 * D_IN, 0.f);
     for (int k = 0; k < D_IN; ++k) norm[D_IN + k] = 1.f;
-    MDET_HIP(hipMemcpy(m->p, p.data(), N_PARAMS * 4, hipMemcpyHostToDevice));
-    MDET_HIP(hipMemcpy(m->norm, norm.data(), 2 * D_IN * 4, hipMemcpyHostToDevice));
-    MDET_HIP(hipMemset(m->m1, 0, N_PARAMS * 4));
-    MDET_HIP(hipMemset(m->m2, 0, N_PARAMS * 4));
-    hipLaunchKernelGGL(k_quantize, dim3((N_PARAMS + 255) / 256), dim3(256), 0, ctx->stream, m->p, m->wq,
-                       (uint32_t)N_PARAMS);
-    MDET_HIP(hipStreamSynchronize(ctx->stream));
+    if ((rc = mlp_copy(m, m->p, p.data(), N_PARAMS * 4, hipMemcpyHostToDevice)) ||
+        (rc = mlp_copy(m, m->norm, norm.data(), 2 * D_IN * 4, hipMemcpyHostToDevice))) {
+        mdet_mlp_destroy(m);
+        return rc;
+    }
+    MDET_HIP(hipMemsetAsync(m->m1, 0, N_PARAMS * 4, ctx->stream));
+    MDET_HIP(hipMemsetAsync(m->m2, 0, N_PARAMS * 4, ctx->stream));
+    if ((rc = mlp_quantize(m))) {
+        mdet_mlp_destroy(m);
+        return rc;
+    }
     *out = m;
     return MDET_OK;
 }
@@ -345,26 +376,24 @@ uint64_t mdet_mlp_param_count(void) { return N_PARAMS; }
 
 int mdet_mlp_set_input_norm(mdet_mlp* m, const float* mean136, const float* istd136) {
     if (!m || !mean136 || !istd136) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_set_input_norm: null argument");
-    MDET_HIP(hipMemcpy(m->norm, mean136, D_IN * 4, hipMemcpyHostToDevice));
-    MDET_HIP(hipMemcpy(m->norm + D_IN, istd136, D_IN * 4, hipMemcpyHostToDevice));
+    MDET_TRY(check_ctx(m->ctx));
+    MDET_TRY(mlp_copy(m, m->norm, mean136, D_IN * 4, hipMemcpyHostToDevice));
+    MDET_TRY(mlp_copy(m, m->norm + D_IN, istd136, D_IN * 4, hipMemcpyHostToDevice));
     return MDET_OK;
 }
 
 int mdet_mlp_get_params(mdet_mlp* m, float* out, uint64_t count) {
     if (!m || !out || count < N_PARAMS) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_get_params: need %d floats", N_PARAMS);
-    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
-    MDET_HIP(hipMemcpy(out, m->p, N_PARAMS * 4, hipMemcpyDeviceToHost));
+    MDET_TRY(mlp_copy(m, out, m->p, N_PARAMS * 4, hipMemcpyDeviceToHost));
+    // with room for twice the parameters, also the copy the forward pass reads
+    if (count >= 2 * (uint64_t)N_PARAMS) MDET_TRY(mlp_copy(m, out + N_PARAMS, m->wq, N_PARAMS * 4, hipMemcpyDeviceToHost));
     return MDET_OK;
 }
 
 int mdet_mlp_set_params(mdet_mlp* m, const float* in, uint64_t count) {
     if (!m || !in || count != N_PARAMS) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_set_params: need %d floats", N_PARAMS);
-    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
-    MDET_HIP(hipMemcpy(m->p, in, N_PARAMS * 4, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_quantize, dim3((N_PARAMS + 255) / 256), dim3(256), 0, m->ctx->stream, m->p, m->wq,
-                       (uint32_t)N_PARAMS);
-    MDET_HIP(hipStreamSynchronize(m->ctx->stream));
-    return MDET_OK;
+    MDET_TRY(mlp_copy(m, m->p, in, N_PARAMS * 4, hipMemcpyHostToDevice));
+    return mlp_quantize(m);
 }
 
 int mdet_mlp_grad(mdet_mlp* m, const float* tnf_dev, const uint32_t* pi_dev, const uint32_t* pj_dev,
@@ -375,9 +404,9 @@ int mdet_mlp_grad(mdet_mlp* m, const float* tnf_dev, const uint32_t* pi_dev, con
         MDET_FAIL(MDET_E_INVALID, "mdet_mlp_grad: empty batch or null argument");
     Scratch S;
     MDET_TRY(mlp_grad(m, tnf_dev, pi_dev, pj_dev, label_dev, batch, margin, &S));
+    if (grad_out) MDET_TRY(mlp_copy(m, grad_out, m->g, N_PARAMS * 4, hipMemcpyDeviceToHost));
+    if (loss_out) MDET_TRY(mlp_copy(m, loss_out, S.loss, 4, hipMemcpyDeviceToHost));
     MDET_HIP(hipStreamSynchronize(m->ctx->stream));
-    if (grad_out) MDET_HIP(hipMemcpy(grad_out, m->g, N_PARAMS * 4, hipMemcpyDeviceToHost));
-    if (loss_out) MDET_HIP(hipMemcpy(loss_out, S.loss, 4, hipMemcpyDeviceToHost));
     return MDET_OK;
 }
 
