@@ -9,13 +9,22 @@ MFMA Gram prefilter + FP32 refinement; the exact FP32 mode and the FP16 mode
 run in the same process, and their edge lists must be identical.
 
 value = n(n-1)/2 / (seconds per step): effective pair evaluations per second.
+
+Also reported: the learned proposal filter (MLP 136-256-256-32 trained on the
+device for 20k steps on 10M pairs labelled by the exact path, embedding radius
+calibrated on the positives) with its edge recall against the exhaustive edge
+set; the counting kernel's HBM throughput; CPU baselines on the host cores
+(the reference's own openmp/TNF.cpp for counting when it is built next to the
+oracle, the oracle's C restatement of gen_tnf_graph for pairs).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import threading
 import time
 
@@ -25,12 +34,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import mdet_amd as M  # noqa: E402
+from mdet_amd import learned as L  # noqa: E402
 from mdet_amd import workload as W  # noqa: E402
 
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16/FP16 MFMA
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 FLOPS_PER_PAIR = 2 * 136   # algorithmic: one K=136 dot product per unordered pair
 CUTOFF, MAX_EDGES = 0.95, 200
+# Memory-side bytes per launch of the BF16 prefilter at the 200,000-contig
+# workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) scaled by
+# the same counters on a plain 1 GiB copy (x2.0 fetch, x1.0 write); summary and
+# commands in profiles/pairs_200k_pmc_traffic.md.  Counters cannot be read in
+# the timed run, so the figure is carried here for the workload it was taken on.
+PMC_TRAFFIC = {"c3_200k": {"fetch_kib": 9275448.6, "write_kib": 164518.1, "fetch_scale": 2.0, "write_scale": 1.0}}
 
 
 def log(msg):
@@ -54,6 +71,121 @@ def run_mode(ctx, w, mode, steps, warmup):
 def fetch_edges(ctx, w, mode):
     f, t, s, st = ctx.tnf_edges(w.tnf, w.logsize, w.n, CUTOFF, MAX_EDGES, mode)
     return f, t, s, st
+
+
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def count_algorithmic_bytes(w):
+    """Bytes the counting step must move: the 2-bit stream, the N-run list,
+    the FP32 rows and the integer counts it writes."""
+    return (w.n_bases + 3) // 4 + 16 * w.n_runs + 8 * (w.n + 1) + 2 * 136 * 4 * w.n
+
+
+def time_count(ctx, w, reps=5):
+    """Wall time of mdet_tnf_count on HBM-resident input (device-synchronised)."""
+    out = []
+    for _ in range(reps + 1):
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.tnf_count(w.packed, w.n_bases, w.runs, w.n_runs, w.base_off, w.n, w.counts, w.tnf)
+        ctx.sync()
+        out.append(time.perf_counter() - t0)
+    return float(np.median(out[1:]))
+
+
+def cpu_count_baseline(w, cfg_name, budget_contigs=30000):
+    """openmp/TNF.cpp (built from the reference sources next to the oracle) on
+    a FASTA of the first contigs of the same workload; its own timer around
+    the TNF loop, which covers contigs of at least 2500 bases."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "tnf_ref")
+    if not os.path.exists(ref):
+        return None
+    cfg = W.gen_config_of(cfg_name, seed=1)
+    m = min(w.n, budget_contigs)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    with tempfile.TemporaryDirectory() as td:
+        fa = os.path.join(td, "sample.fa")
+        with open(fa, "wb") as f:
+            for c in range(m):
+                f.write(b">c%d\n" % c)
+                f.write(M.gen_ascii(cfg, c, int(w.lengths[c])))
+                f.write(b"\n")
+        r = subprocess.run([ref, str(threads), fa], cwd=td, capture_output=True, text=True, timeout=600)
+    secs = None
+    for line in r.stdout.splitlines():
+        if line.startswith("Crear matriz TNF"):
+            secs = float(line.split()[-1].rstrip("s"))
+    if r.returncode != 0 or secs is None:
+        return None
+    lens = w.lengths[:m]
+    bases = int(lens[lens >= 2500].sum())
+    return {"value": bases / secs, "unit": "bases/s", "cores": threads, "kind": "reference",
+            "sample": f"first {m} contigs of the workload as FASTA ({bases} bases in contigs >= 2500, the ones "
+                      f"openmp/TNF.cpp counts), its own timer around the TNF loop: {secs:.3f} s"}
+
+
+def run_learned(ctx, w, fb, tb, train_steps, steps, quantile=0.9995):
+    """Train the filter on pairs the exact path labels, then time the learned
+    edge list and measure its recall against the exhaustive edge set."""
+    n = w.n
+    t0 = time.perf_counter()
+    pi, pj, y, pos = L.training_set(ctx, w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, n_pairs=10_000_000, seed=1)
+    set_s = time.perf_counter() - t0
+    mean, istd = L.input_norm(w.tnf.numpy())
+    mlp = M.Mlp(ctx, seed=7)
+    mlp.set_input_norm(mean, istd)
+    curve, train_s = L.train(ctx, mlp, w.tnf, pi, pj, y, train_steps, batch=512, lr=2e-3, margin=4.0,
+                             log_every=max(1, train_steps // 10))
+    emb = L.embeddings(ctx, mlp, w.tnf, n)
+    r2 = L.calibrate_radius2(emb, pos["i"], pos["j"], quantile=quantile)
+    for _ in range(1):
+        ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, mlp=mlp, radius2=r2)
+    ctx.sync()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, mlp=mlp, radius2=r2)
+        stats.append(st.as_dict())
+    ctx.sync()
+    sec = (time.perf_counter() - t0) / steps
+    f, t, s, st = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, mlp=mlp, radius2=r2)
+    exact = fb.astype(np.uint64) * n + tb.astype(np.uint64)
+    got = f.astype(np.uint64) * n + t.astype(np.uint64)
+    hit = np.isin(exact, got)
+    mlp.close()
+    return {"pairs_per_s": n * (n - 1) // 2 / sec, "ms_per_step": sec * 1e3,
+            "stage_ms": {k: float(np.mean([x[k] for x in stats]))
+                         for k in ("ms_prefilter", "ms_refine", "ms_select", "ms_total")},
+            "survivors_exact_evaluated": stats[-1]["survivors"], "edges": int(len(f)),
+            "edge_recall": float(hit.mean()) if len(exact) else 1.0,
+            "edges_not_in_exhaustive_set": int((~np.isin(got, exact)).sum()),
+            "radius2": r2, "radius_quantile": quantile,
+            "training": {"pairs": int(len(pi)), "positives": int(y.sum()), "steps": train_steps, "batch": 512,
+                         "seconds": train_s, "ms_per_step": 1e3 * train_s / max(1, train_steps),
+                         "set_build_s": set_s, "loss_curve": [(int(a), float(b)) for a, b in curve]}}
+
+
+def copy_probe(mib):
+    """A plain device copy of known size (elementwise y = 2x, mib MiB each way),
+    for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": mib << 20, "kernel": "elementwise mul"}))
 
 
 def cpu_baseline(tnf, ls, budget_s=12.0):
@@ -100,16 +232,31 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="override the contig count")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="exact_f32,f16_refine")
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain copy of this size (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
 
     ctx = M.Ctx(0)
     log(f"generating {args.workload}")
-    w = W.make(ctx, args.workload, n=args.n, seed=1)
+    w = W.make(ctx, args.workload, n=args.n, seed=1, keep_stream=True)
     log(f"{w.n} contigs, {w.n_bases} bases, generated in {w.gen_s:.1f} s, counted in {w.count_ms:.1f} ms")
     P = w.n * (w.n - 1) // 2
+    count_s = time_count(ctx, w)
+    count_bytes = count_algorithmic_bytes(w)
+    count = {"bases_per_s": w.n_bases / count_s, "ms": count_s * 1e3, "algorithmic_bytes": count_bytes,
+             "achieved_GBs": count_bytes / count_s / 1e9, "peak_GBs": PEAK_HBM_GBS,
+             "frac": count_bytes / count_s / 1e9 / PEAK_HBM_GBS,
+             "note": "wall time of mdet_tnf_count (counting + normalisation kernels, host checks) on resident input"}
+    for a in (w.packed, w.runs, w.base_off):
+        a.free()
 
     # headline: BF16 MFMA prefilter + FP32 refine
     log("timing bf16_refine")
@@ -139,18 +286,35 @@ def main():
                        "max_abs_s_diff_vs_bf16": float(np.max(np.abs(s - sb))) if same and len(s) else None}
     if "exact_f32" in modes:
         modes["bf16_refine"]["speedup_vs_exact_f32"] = modes["bf16_refine"]["pairs_per_s"] / modes["exact_f32"]["pairs_per_s"]
+    if not args.no_learned:
+        log(f"learned filter: {args.train_steps} training steps")
+        lm = run_learned(ctx, w, fb, tb, args.train_steps, max(1, args.steps // 3))
+        lm["speedup_vs_bf16_refine"] = lm["pairs_per_s"] / modes["bf16_refine"]["pairs_per_s"]
+        modes["learned"] = lm
 
     achieved = FLOPS_PER_PAIR * P / (pf_ms * 1e-3) / 1e12
+    traffic = None
+    pm = PMC_TRAFFIC.get(args.workload) if args.n is None else None
+    if pm:
+        traffic = 1024 * (pm["fetch_kib"] * pm["fetch_scale"] + pm["write_kib"] * pm["write_scale"])
     roofline = {"bound": "mfma", "kernel": "k_prefilter_mfma<bf16>", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
-                "flops_per_launch": FLOPS_PER_PAIR * P, "avg_launch_ms": pf_ms}
+                "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": traffic,
+                "traffic_unit": "bytes per launch (memory side, rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                "flops_per_launch": FLOPS_PER_PAIR * P, "avg_launch_ms": pf_ms,
+                "timing": "HIP events recorded on the context stream around the launch, mean of the timed steps"}
 
     cpu = None
+    cpu_count = None
     parity = {}
     if not args.no_cpu:
         log("cpu baseline (oracle, row sample)")
         tnf, ls = w.tnf.numpy(), w.logsize.numpy()
         cpu, (fo, to), rows = cpu_baseline(tnf, ls)
+        cpu.update(host_info())
+        log("cpu counting baseline (openmp/TNF.cpp)")
+        cpu_count = cpu_count_baseline(w, args.workload)
+        if cpu_count:
+            cpu_count.update(host_info())
         m = fb < rows
         parity = {"oracle_rows_checked": rows,
                   "edges_in_rows_equal_oracle": bool(set(zip(fo.tolist(), to.tolist())) ==
@@ -173,12 +337,14 @@ def main():
                                f"{w.n_bases} bases; cutoff {CUTOFF}, maxEdges {MAX_EDGES}; step = full edge list "
                                f"of {P} pairs (bf16_refine)",
                    "n_contigs": w.n, "pairs": P, "cutoff": CUTOFF, "max_edges": MAX_EDGES,
-                   "count_ms": w.count_ms, "count_bases_per_s": w.n_bases / (w.count_ms * 1e-3)},
+                   "count_ms": count["ms"], "count_bases_per_s": count["bases_per_s"]},
         "modes": modes,
+        "count": count,
         "edge_sets_identical": identical,
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_count": cpu_count,
     }
     print(json.dumps(line))
     ctx.close()

@@ -268,7 +268,23 @@ int mdet_gen_ascii(const mdet_gen_config* cfg, uint64_t contig, char* out, uint6
     if (len != h.len) MDET_FAIL(MDET_E_INVALID, "mdet_gen_ascii: contig %llu has length %llu, not %llu",
                                 (unsigned long long)contig, (unsigned long long)h.len, (unsigned long long)len);
     if (!out && len) MDET_FAIL(MDET_E_INVALID, "mdet_gen_ascii: null output");
-    std::vector<Genome> genomes = make_genomes(*cfg);
+    // the genome tables depend on (seed, n_genomes, gc range) only; callers
+    // write a FASTA one contig at a time, so keep the last set per thread
+    thread_local struct {
+        uint64_t seed = 0;
+        uint32_t n_genomes = 0;
+        double gc_lo = 0, gc_hi = 0;
+        std::vector<Genome> g;
+    } cache;
+    if (cache.g.empty() || cache.seed != cfg->seed || cache.n_genomes != cfg->n_genomes || cache.gc_lo != cfg->gc_lo ||
+        cache.gc_hi != cfg->gc_hi) {
+        cache.g = make_genomes(*cfg);
+        cache.seed = cfg->seed;
+        cache.n_genomes = cfg->n_genomes;
+        cache.gc_lo = cfg->gc_lo;
+        cache.gc_hi = cfg->gc_hi;
+    }
+    const std::vector<Genome>& genomes = cache.g;
     static const char B[5] = {'A', 'C', 'G', 'T', 'N'};
     gen_contig(*cfg, genomes, contig, h, [&](uint64_t p, unsigned b) { out[p] = B[b]; });
     return MDET_OK;

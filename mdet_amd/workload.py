@@ -47,12 +47,17 @@ class Workload:
     count_ms: float
 
 
+def gen_config_of(name: str, seed: int = 1):
+    """Generator configuration of a named workload (contig i of the workload is
+    contig i of this configuration, whatever the contig count)."""
+    p = CONFIGS[name]
+    return gen_config(seed=seed, n_genomes=p["n_genomes"], len_median=p["len_median"], len_sigma=p["len_sigma"],
+                      len_min=p["len_min"], len_max=p["len_max"], n_frac=p["n_frac"])
+
+
 def make(ctx: Ctx, name: str, n: int | None = None, seed: int = 1, keep_stream: bool = False) -> Workload:
-    p = dict(CONFIGS[name])
-    n = p.pop("n") if n is None else n
-    p.pop("n", None)
-    cfg = gen_config(seed=seed, n_genomes=p["n_genomes"], len_median=p["len_median"], len_sigma=p["len_sigma"],
-                     len_min=p["len_min"], len_max=p["len_max"], n_frac=p["n_frac"])
+    n = CONFIGS[name]["n"] if n is None else n
+    cfg = gen_config_of(name, seed)
     t0 = time.time()
     packed, runs, off, lens, gid = gen_packed(cfg, n)
     gen_s = time.time() - t0
