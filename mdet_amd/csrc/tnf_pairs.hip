@@ -780,23 +780,99 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
 }
 
 // ---------------------------------------------------------------- select --
-__global__ void k_degree(const Cand* __restrict__ cand, uint64_t nc, uint32_t* __restrict__ deg) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nc) return;
-    const Cand c = cand[k];
-    if (!(c.flags & 2u)) return;
-    atomicAdd(&deg[c.i], 1u);
-    atomicAdd(&deg[c.j], 1u);
+// Kept candidates -> per row the max_edges largest preProb -> edges i < j,
+// sorted by (i, j).  Rows are handled in buckets of S_BUCKET consecutive
+// positions of the prefilter's row order (the composition order of the MFMA
+// modes, contig order for the FP32 mode).  A refined survivor run shares one
+// row bucket and spans a few column buckets, so the bucket counts and slot
+// reservations are aggregated per wave (one device atomic per distinct bucket
+// among a wave's 64 candidates, instead of one per candidate endpoint) and a
+// bucket's entries are written as a few contiguous runs.  One block per
+// bucket then groups its entries by row with LDS atomics, and one wave per
+// row selects and sorts in LDS; rows longer than W_CAP entries, and rows
+// whose FP32 tie zone at the cut exceeds one wave, take the block-wide path.
+constexpr int S_BUCKET = 256;
+constexpr int S_THREADS = 512;          // 8 waves per bucket block
+constexpr int S_WAVES = S_THREADS / 64;
+constexpr int W_CAP = 512;              // entries of a row sorted by one wave
+
+struct Ent {
+    uint32_t partner;
+    float pre;
+    uint32_t lrow;  // row within the bucket
+};
+
+__device__ inline uint32_t position_of(const uint32_t* __restrict__ iperm, uint32_t c) {
+    return iperm ? iperm[c] : c;
 }
 
-__global__ void k_scatter(const Cand* __restrict__ cand, uint64_t nc, const uint64_t* __restrict__ rowptr,
-                          uint32_t* __restrict__ cursor, Adj* __restrict__ adj) {
+// Adds, for each distinct key held by the active lanes, the number of lanes
+// holding it to counter[key] (one atomic by the lowest such lane) and returns
+// to every active lane old value + its rank among the lanes with its key.
+// All 64 lanes call; the loop runs once per distinct key.
+__device__ inline uint32_t wave_aggregated_add(uint32_t* counter, uint32_t key, bool active) {
+    uint32_t slot = 0;
+    uint64_t pending = __ballot(active);
+    const uint32_t lane = lane_id();
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t k = __shfl(key, leader, 64);
+        const bool mine = active && key == k;
+        const uint64_t same = __ballot(mine);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counter[k], (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if (mine) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
+
+__global__ __launch_bounds__(256) void k_iperm(const uint32_t* __restrict__ perm, uint32_t n,
+                                               uint32_t* __restrict__ iperm) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) iperm[perm[s]] = s;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_count(const Cand* __restrict__ cand, uint64_t nc,
+                                                      const uint32_t* __restrict__ iperm, uint32_t* __restrict__ bcnt) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nc) return;
-    const Cand c = cand[k];
-    if (!(c.flags & 2u)) return;
-    adj[rowptr[c.i] + atomicAdd(&cursor[c.i], 1u)] = Adj{c.j, c.pre};
-    adj[rowptr[c.j] + atomicAdd(&cursor[c.j], 1u)] = Adj{c.i, c.pre};
+    bool act = false;
+    uint32_t bi = 0, bj = 0;
+    if (k < nc) {
+        const Cand c = cand[k];
+        if (c.flags & 2u) {
+            act = true;
+            bi = position_of(iperm, c.i) / S_BUCKET;
+            bj = position_of(iperm, c.j) / S_BUCKET;
+        }
+    }
+    (void)wave_aggregated_add(bcnt, bi, act);
+    (void)wave_aggregated_add(bcnt, bj, act);
+}
+
+__global__ __launch_bounds__(256) void k_bucket_fill(const Cand* __restrict__ cand, uint64_t nc,
+                                                     const uint32_t* __restrict__ iperm,
+                                                     const uint64_t* __restrict__ boff, uint32_t* __restrict__ bcur,
+                                                     Ent* __restrict__ ent) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false;
+    Cand c{0, 0, 0.f, 0};
+    uint32_t si = 0, sj = 0;
+    if (k < nc) {
+        c = cand[k];
+        if (c.flags & 2u) {
+            act = true;
+            si = position_of(iperm, c.i);
+            sj = position_of(iperm, c.j);
+        }
+    }
+    const uint32_t s1 = wave_aggregated_add(bcur, si / S_BUCKET, act);
+    const uint32_t s2 = wave_aggregated_add(bcur, sj / S_BUCKET, act);
+    if (act) {
+        ent[boff[si / S_BUCKET] + s1] = Ent{c.j, c.pre, si % S_BUCKET};
+        ent[boff[sj / S_BUCKET] + s2] = Ent{c.i, c.pre, sj % S_BUCKET};
+    }
 }
 
 // (pre desc, partner asc): true if x ranks before y
@@ -804,42 +880,55 @@ __device__ inline bool ranks_before(float px, uint32_t ix, float py, uint32_t iy
     return px > py || (px == py && ix < iy);
 }
 
-// Bitonic sort of sz (power of two) entries in LDS by ranks_before.
-__device__ void block_sort_rank(float* key, uint32_t* id, int sz) {
+// Orders the LDS writes of a wave before its following LDS reads (the lanes
+// of one wave execute each LDS instruction together and in program order; the
+// fences keep the compiler from moving accesses across).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bitonic sorts of sz (power of two) LDS entries by one wave (BY_RANK: by
+// ranks_before; else by id ascending), the key carried along.
+template <bool BY_RANK>
+__device__ void wave_sort(float* key, uint32_t* id, int sz, int lane) {
     for (int k = 2; k <= sz; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = threadIdx.x; t < sz; t += blockDim.x) {
+            for (int t = lane; t < sz; t += 64) {
                 const int u = t ^ j;
                 if (u > t) {
                     const bool up = (t & k) == 0;
-                    const bool before = ranks_before(key[u], id[u], key[t], id[t]);
+                    const bool before = BY_RANK ? ranks_before(key[u], id[u], key[t], id[t]) : (id[u] < id[t]);
                     if (before == up) {
-                        float tk = key[t];
+                        const float tk = key[t];
                         key[t] = key[u];
                         key[u] = tk;
-                        uint32_t ti = id[t];
+                        const uint32_t ti = id[t];
                         id[t] = id[u];
                         id[u] = ti;
                     }
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
 }
 
-// Bitonic sort of sz entries by id ascending (key carried along).
-__device__ void block_sort_id(float* key, uint32_t* id, int sz) {
+// The same by a whole block (blockDim.x threads).
+template <bool BY_RANK>
+__device__ void block_sort(float* key, uint32_t* id, int sz) {
     for (int k = 2; k <= sz; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int t = threadIdx.x; t < sz; t += blockDim.x) {
                 const int u = t ^ j;
                 if (u > t) {
                     const bool up = (t & k) == 0;
-                    if ((id[u] < id[t]) == up) {
-                        float tk = key[t];
+                    const bool before = BY_RANK ? ranks_before(key[u], id[u], key[t], id[t]) : (id[u] < id[t]);
+                    if (before == up) {
+                        const float tk = key[t];
                         key[t] = key[u];
                         key[u] = tk;
-                        uint32_t ti = id[t];
+                        const uint32_t ti = id[t];
                         id[t] = id[u];
                         id[u] = ti;
                     }
@@ -849,82 +938,155 @@ __device__ void block_sort_id(float* key, uint32_t* id, int sz) {
         }
 }
 
-// One block per row: pick the max_edges best partners (all if max_edges == 0
-// or fewer candidates), keep partners > row, sort them, write them back to the
-// front of the row's segment and their count to out_cnt[row].
-// err[0] |= 1: a row over SEL_CAP; err[0] |= 2: a tie zone over ZONE_CAP.
-constexpr int S_THREADS = 256;
-__global__ __launch_bounds__(S_THREADS) void k_select(const uint64_t* __restrict__ rowptr, Adj* __restrict__ adj,
-                                                      uint32_t n, uint32_t K, const float* __restrict__ tnf,
-                                                      const double* __restrict__ ls, uint32_t* __restrict__ out_cnt,
-                                                      unsigned int* __restrict__ err,
-                                                      unsigned long long* __restrict__ capped) {
-    __shared__ float key[SEL_CAP];
-    __shared__ uint32_t id[SEL_CAP];
-    __shared__ double zkey[ZONE_CAP];
-    __shared__ int s_z[2];
-    __shared__ uint32_t s_cnt;
-    const uint32_t row = blockIdx.x;
-    const uint64_t beg = rowptr[row];
-    const uint32_t deg = (uint32_t)(rowptr[row + 1] - beg);
-    if (deg == 0) {
-        if (threadIdx.x == 0) out_cnt[row] = 0;
-        return;
+__device__ inline int pow2_at_least(uint32_t v) {
+    int sz = 1;
+    while (sz < (int)v) sz <<= 1;
+    return sz;
+}
+
+// Row `rid` (contig id), deg entries at adj[beg..): by one wave.  Returns
+// false, touching nothing in global memory, if the row needs the block path
+// (FP32 tie zone at the cut wider than a wave).
+__device__ bool wave_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t deg, uint32_t rid, uint32_t K,
+                                const float* __restrict__ tnf, const double* __restrict__ ls, float* key,
+                                uint32_t* id, uint32_t* __restrict__ out_cnt, uint64_t* __restrict__ rbeg,
+                                unsigned long long* __restrict__ capped, int lane) {
+    const bool cap = K && deg > K;
+    int sz;
+    if (cap) {
+        sz = pow2_at_least(deg);
+        for (int t = lane; t < sz; t += 64) {
+            const bool in = t < (int)deg;
+            const Adj e = in ? adj[beg + t] : Adj{0xffffffffu, 0.f};
+            key[t] = in ? e.pre : -__builtin_inff();
+            id[t] = in ? e.partner : 0xffffffffu;
+        }
+        wave_sync();
+        wave_sort<true>(key, id, sz, lane);
+        const float vk = key[K - 1];
+        // the FP32 keys within 2*EPS of the K-th form a contiguous range [z0, z1)
+        int z0 = (int)deg, z1 = 0;
+        for (int t0 = 0; t0 < (int)deg; t0 += 64) {
+            const int t = t0 + lane;
+            const uint64_t m = __ballot(t < (int)deg && fabsf(key[t] - vk) <= 2.f * EPS_PRE);
+            if (m) {
+                z0 = min(z0, t0 + __ffsll((long long)m) - 1);
+                z1 = max(z1, t0 + 64 - __clzll((long long)m));
+            }
+        }
+        const int zn = z1 - z0;
+        if (zn > 64) return false;
+        if (zn > 1) {
+            double zk = 0.0;
+            uint32_t myid = 0xffffffffu;
+            float mykey = 0.f;
+            if (lane < zn) {
+                myid = id[z0 + lane];
+                mykey = key[z0 + lane];
+                zk = model64(tnf + (size_t)rid * NT, tnf + (size_t)myid * NT, ls[rid], ls[myid]);
+            }
+            int rk = 0;
+            for (int u = 0; u < zn; ++u) {
+                const double zu = __shfl(zk, u, 64);
+                const uint32_t iu = __shfl(myid, u, 64);
+                rk += (zu > zk || (zu == zk && iu < myid)) ? 1 : 0;
+            }
+            wave_sync();
+            if (lane < zn) {
+                id[z0 + rk] = myid;
+                key[z0 + rk] = mykey;
+            }
+            wave_sync();
+        }
+        if (lane == 0) atomicAdd(capped, 1ull);
+        // keep the K selected partners above the row, then sort them by id
+        for (int t = lane; t < sz; t += 64)
+            if (t >= (int)K || id[t] <= rid) id[t] = 0xffffffffu;
+        wave_sync();
+    } else {
+        // every candidate is selected: gather the partners above the row
+        uint32_t cnt = 0;
+        for (uint32_t t0 = 0; t0 < deg; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            Adj e{0, 0.f};
+            if (t < deg) e = adj[beg + t];
+            const bool keep = t < deg && e.partner > rid;
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                const uint32_t o = cnt + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                key[o] = e.pre;
+                id[o] = e.partner;
+            }
+            cnt += (uint32_t)__popcll(m);
+        }
+        sz = pow2_at_least(cnt);
+        for (int t = (int)cnt + lane; t < sz; t += 64) id[t] = 0xffffffffu;
+        wave_sync();
     }
+    wave_sort<false>(key, id, sz, lane);
+    uint32_t cnt = 0;
+    for (int t0 = 0; t0 < sz; t0 += 64) cnt += (uint32_t)__popcll(__ballot(t0 + lane < sz && id[t0 + lane] != 0xffffffffu));
+    for (uint32_t t = lane; t < cnt; t += 64) adj[beg + t] = Adj{id[t], key[t]};
+    if (lane == 0) {
+        out_cnt[rid] = cnt;
+        rbeg[rid] = beg;
+    }
+    return true;
+}
+
+// The same by the whole block, for rows of up to SEL_CAP entries and tie
+// zones of up to ZONE_CAP entries (err bits 1 and 2 otherwise).
+__device__ void block_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t deg, uint32_t rid, uint32_t K,
+                                 const float* __restrict__ tnf, const double* __restrict__ ls, float* key,
+                                 uint32_t* id, double* zkey, int* s_z, uint32_t* s_cnt,
+                                 uint32_t* __restrict__ out_cnt, uint64_t* __restrict__ rbeg,
+                                 unsigned int* __restrict__ err, unsigned long long* __restrict__ capped) {
     if (deg > SEL_CAP) {
         if (threadIdx.x == 0) {
             atomicOr(err, 1u);
-            out_cnt[row] = 0;
+            out_cnt[rid] = 0;
+            rbeg[rid] = beg;
         }
         return;
     }
-    int sz = 1;
-    while (sz < (int)deg) sz <<= 1;
+    const int sz = pow2_at_least(deg);
     for (int t = threadIdx.x; t < sz; t += blockDim.x) {
-        if (t < (int)deg) {
-            const Adj e = adj[beg + t];
-            key[t] = e.pre;
-            id[t] = e.partner;
-        } else {
-            key[t] = -__builtin_inff();
-            id[t] = 0xffffffffu;
-        }
+        const bool in = t < (int)deg;
+        const Adj e = in ? adj[beg + t] : Adj{0xffffffffu, 0.f};
+        key[t] = in ? e.pre : -__builtin_inff();
+        id[t] = in ? e.partner : 0xffffffffu;
     }
     __syncthreads();
     uint32_t nsel = deg;
     if (K && deg > K) {
         if (threadIdx.x == 0) atomicAdd(capped, 1ull);
-        block_sort_rank(key, id, sz);
+        block_sort<true>(key, id, sz);
         const float vk = key[K - 1];
-        // zone of FP32 keys within 2*EPS of the K-th: [z0, z1) in sorted order
         if (threadIdx.x == 0) {
             s_z[0] = (int)deg;
             s_z[1] = 0;
         }
         __syncthreads();
-        for (int t = threadIdx.x; t < (int)deg; t += blockDim.x) {
-            const bool inz = fabsf(key[t] - vk) <= 2.f * EPS_PRE;
-            if (inz) {
+        for (int t = threadIdx.x; t < (int)deg; t += blockDim.x)
+            if (fabsf(key[t] - vk) <= 2.f * EPS_PRE) {
                 atomicMin(&s_z[0], t);
                 atomicMax(&s_z[1], t + 1);
             }
-        }
         __syncthreads();
         const int z0 = s_z[0], z1 = s_z[1], zn = z1 - z0;
         if (zn > 1) {
             if (zn > ZONE_CAP) {
                 if (threadIdx.x == 0) atomicOr(err, 2u);
             } else {
-                const float* a = tnf + (size_t)row * NT;
                 for (int t = threadIdx.x; t < zn; t += blockDim.x) {
                     const uint32_t p = id[z0 + t];
-                    zkey[t] = model64(a, tnf + (size_t)p * NT, ls[row], ls[p]);
+                    zkey[t] = model64(tnf + (size_t)rid * NT, tnf + (size_t)p * NT, ls[rid], ls[p]);
                 }
                 __syncthreads();
                 // rank each zone entry by (FP64 key desc, partner asc)
-                uint32_t myid[2];
-                float mykey[2];
-                int myrank[2];
+                uint32_t myid[ZONE_CAP / S_THREADS + 1];
+                float mykey[ZONE_CAP / S_THREADS + 1];
+                int myrank[ZONE_CAP / S_THREADS + 1];
                 int cntm = 0;
                 for (int t = threadIdx.x; t < zn; t += blockDim.x, ++cntm) {
                     int rk = 0;
@@ -944,28 +1106,115 @@ __global__ __launch_bounds__(S_THREADS) void k_select(const uint64_t* __restrict
         }
         nsel = K;
     }
-    // keep the selected partners > row, sorted ascending
     for (int t = threadIdx.x; t < sz; t += blockDim.x)
-        if (t >= (int)nsel || id[t] <= row) id[t] = 0xffffffffu;
-    if (threadIdx.x == 0) s_cnt = 0;
+        if (t >= (int)nsel || id[t] <= rid) id[t] = 0xffffffffu;
+    if (threadIdx.x == 0) *s_cnt = 0;
     __syncthreads();
-    block_sort_id(key, id, sz);
+    block_sort<false>(key, id, sz);
     for (int t = threadIdx.x; t < sz; t += blockDim.x)
-        if (id[t] != 0xffffffffu) atomicAdd(&s_cnt, 1u);
+        if (id[t] != 0xffffffffu) atomicAdd(s_cnt, 1u);
     __syncthreads();
-    const uint32_t cnt = s_cnt;
+    const uint32_t cnt = *s_cnt;
     for (int t = threadIdx.x; t < (int)cnt; t += blockDim.x) adj[beg + t] = Adj{id[t], key[t]};
-    if (threadIdx.x == 0) out_cnt[row] = cnt;
+    if (threadIdx.x == 0) {
+        out_cnt[rid] = cnt;
+        rbeg[rid] = beg;
+    }
+    __syncthreads();
 }
 
-__global__ void k_emit(const uint64_t* __restrict__ rowptr, const Adj* __restrict__ adj,
+// One block per bucket: group the bucket's entries by row into adj (same
+// offsets as ent), then select every row.  Output per contig row: out_cnt
+// (edges kept, partners above the row, sorted) and rbeg (where they are in adj).
+__global__ __launch_bounds__(S_THREADS) void k_bucket_select(
+    const uint64_t* __restrict__ boff, const Ent* __restrict__ ent, Adj* __restrict__ adj, uint32_t n, uint32_t K,
+    const uint32_t* __restrict__ perm, const float* __restrict__ tnf, const double* __restrict__ ls,
+    uint32_t* __restrict__ out_cnt, uint64_t* __restrict__ rbeg, unsigned int* __restrict__ err,
+    unsigned long long* __restrict__ capped) {
+    __shared__ uint32_t hist[S_BUCKET], rstart[S_BUCKET], rcur[S_BUCKET];
+    __shared__ float wkey[S_WAVES][W_CAP];
+    __shared__ uint32_t wid[S_WAVES][W_CAP];
+    __shared__ float bkey[SEL_CAP];
+    __shared__ uint32_t bid[SEL_CAP];
+    __shared__ double zkey[ZONE_CAP];
+    __shared__ uint32_t big[S_BUCKET];
+    __shared__ uint32_t nbig, s_cnt;
+    __shared__ int s_z[2];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t b = blockIdx.x;
+    const uint64_t e0 = boff[b], e1 = boff[b + 1];
+    const uint32_t row0 = b * S_BUCKET;
+    const uint32_t rows = n - row0 < (uint32_t)S_BUCKET ? n - row0 : (uint32_t)S_BUCKET;
+    for (int t = tid; t < S_BUCKET; t += S_THREADS) hist[t] = rcur[t] = 0;
+    if (tid == 0) nbig = 0;
+    __syncthreads();
+    for (uint64_t e = e0 + tid; e < e1; e += S_THREADS) atomicAdd(&hist[ent[e].lrow], 1u);
+    __syncthreads();
+    if (wave == 0) {  // exclusive scan of the 256 row counts, 4 per lane
+        uint32_t v[4], s = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = hist[4 * lane + q];
+            s += v[q];
+        }
+        uint32_t x = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        uint32_t run = x - s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            rstart[4 * lane + q] = run;
+            run += v[q];
+        }
+    }
+    __syncthreads();
+    for (uint64_t e = e0 + tid; e < e1; e += S_THREADS) {
+        const Ent x = ent[e];
+        const uint32_t s = atomicAdd(&rcur[x.lrow], 1u);
+        adj[e0 + rstart[x.lrow] + s] = Adj{x.partner, x.pre};
+    }
+    __syncthreads();
+    for (uint32_t r = wave; r < rows; r += S_WAVES) {
+        const uint32_t deg = hist[r];
+        const uint64_t beg = e0 + rstart[r];
+        const uint32_t pos = row0 + r;
+        const uint32_t rid = perm ? perm[pos] : pos;
+        bool done = true;
+        if (deg == 0) {
+            if (lane == 0) {
+                out_cnt[rid] = 0;
+                rbeg[rid] = beg;
+            }
+        } else if (deg > (uint32_t)W_CAP) {
+            done = false;
+        } else {
+            done = wave_row_select(adj, beg, deg, rid, K, tnf, ls, wkey[wave], wid[wave], out_cnt, rbeg, capped, lane);
+        }
+        if (!done && lane == 0) big[atomicAdd(&nbig, 1u)] = r;
+        wave_sync();
+    }
+    __syncthreads();
+    const uint32_t nb = nbig;
+    for (uint32_t q = 0; q < nb; ++q) {
+        const uint32_t r = big[q];
+        const uint32_t pos = row0 + r;
+        block_row_select(adj, e0 + rstart[r], hist[r], perm ? perm[pos] : pos, K, tnf, ls, bkey, bid, zkey, s_z,
+                         &s_cnt, out_cnt, rbeg, err, capped);
+    }
+}
+
+// Edges of row r (contig order) to their place eoff[r]; one wave per row.
+__global__ void k_emit(const uint64_t* __restrict__ rbeg, const Adj* __restrict__ adj,
                        const uint32_t* __restrict__ out_cnt, const uint64_t* __restrict__ eoff, uint32_t n,
                        uint32_t* __restrict__ from, uint32_t* __restrict__ to, float* __restrict__ s) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (row >= n) return;
     const uint32_t c = out_cnt[row];
-    const uint64_t src = rowptr[row], dst = eoff[row];
+    const uint64_t src = rbeg[row], dst = eoff[row];
     for (uint32_t t = lane; t < c; t += 64) {
         const Adj e = adj[src + t];
         from[dst + t] = row;
@@ -1428,30 +1677,38 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         st.adjudicated = rch[1];
 
         // -- select
-        uint32_t *deg, *cursor, *out_cnt;
-        uint64_t *rowptr, *eoff;
+        const uint32_t nb = (n + S_BUCKET - 1) / S_BUCKET;
+        uint32_t *bcnt, *bcur, *out_cnt, *iperm = nullptr;
+        uint64_t *boff, *eoff, *rbeg;
+        Ent* ent;
         Adj* adj;
         unsigned int* err;
         unsigned long long* capped;
-        MDET_TRY(ctx_buf(ctx, "sel.deg", (size_t)n + 1, &deg));
-        MDET_TRY(ctx_buf(ctx, "sel.cursor", (size_t)n, &cursor));
+        MDET_TRY(ctx_buf(ctx, "sel.bcnt", (size_t)nb + 1, &bcnt));
+        MDET_TRY(ctx_buf(ctx, "sel.bcur", (size_t)nb, &bcur));
+        MDET_TRY(ctx_buf(ctx, "sel.boff", (size_t)nb + 1, &boff));
         MDET_TRY(ctx_buf(ctx, "sel.outcnt", (size_t)n + 1, &out_cnt));
-        MDET_TRY(ctx_buf(ctx, "sel.rowptr", (size_t)n + 1, &rowptr));
+        MDET_TRY(ctx_buf(ctx, "sel.rbeg", (size_t)n, &rbeg));
         MDET_TRY(ctx_buf(ctx, "sel.eoff", (size_t)n + 1, &eoff));
+        MDET_TRY(ctx_buf(ctx, "sel.ent", std::max<uint64_t>(2 * st.candidates, 1), &ent));
         MDET_TRY(ctx_buf(ctx, "sel.adj", std::max<uint64_t>(2 * st.candidates, 1), &adj));
         MDET_TRY(ctx_buf(ctx, "sel.err", 4, &err));
         capped = reinterpret_cast<unsigned long long*>(err + 2);
-        MDET_HIP(hipMemsetAsync(deg, 0, ((size_t)n + 1) * 4, ctx->stream));
-        MDET_HIP(hipMemsetAsync(cursor, 0, (size_t)n * 4, ctx->stream));
+        MDET_HIP(hipMemsetAsync(bcnt, 0, ((size_t)nb + 1) * 4, ctx->stream));
+        MDET_HIP(hipMemsetAsync(bcur, 0, (size_t)nb * 4, ctx->stream));
         MDET_HIP(hipMemsetAsync(out_cnt, 0, ((size_t)n + 1) * 4, ctx->stream));
         MDET_HIP(hipMemsetAsync(err, 0, 16, ctx->stream));
         if (nc) {
-            hipLaunchKernelGGL(k_degree, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream, cand, nc, deg);
-            MDET_TRY(exclusive_scan_u32_to_u64(ctx, deg, rowptr, n));
-            hipLaunchKernelGGL(k_scatter, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, ctx->stream, cand, nc,
-                               rowptr, cursor, adj);
-            hipLaunchKernelGGL(k_select, dim3(n), dim3(S_THREADS), 0, ctx->stream, rowptr, adj, n, max_edges, tnf_dev,
-                               logsize_dev, out_cnt, err, capped);
+            if (perm) {
+                MDET_TRY(ctx_buf(ctx, "sel.iperm", (size_t)n, &iperm));
+                hipLaunchKernelGGL(k_iperm, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, perm, n, iperm);
+            }
+            const unsigned cblocks = (unsigned)((nc + 255) / 256);
+            hipLaunchKernelGGL(k_bucket_count, dim3(cblocks), dim3(256), 0, ctx->stream, cand, nc, iperm, bcnt);
+            MDET_TRY(exclusive_scan_u32_to_u64(ctx, bcnt, boff, nb));
+            hipLaunchKernelGGL(k_bucket_fill, dim3(cblocks), dim3(256), 0, ctx->stream, cand, nc, iperm, boff, bcur, ent);
+            hipLaunchKernelGGL(k_bucket_select, dim3(nb), dim3(S_THREADS), 0, ctx->stream, boff, ent, adj, n, max_edges,
+                               perm, tnf_dev, logsize_dev, out_cnt, rbeg, err, capped);
             MDET_HIP(hipGetLastError());
             MDET_TRY(exclusive_scan_u32_to_u64(ctx, out_cnt, eoff, n));
             unsigned long long errh[2];
@@ -1474,8 +1731,8 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             MDET_TRY(ctx_buf(ctx, "out.from", std::max<uint64_t>(ne, 1), &from));
             MDET_TRY(ctx_buf(ctx, "out.to", std::max<uint64_t>(ne, 1), &to));
             MDET_TRY(ctx_buf(ctx, "out.s", std::max<uint64_t>(ne, 1), &s));
-            hipLaunchKernelGGL(k_emit, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, rowptr, adj, out_cnt, eoff, n,
-                               from, to, s);
+            hipLaunchKernelGGL(k_emit, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, rbeg, adj, out_cnt, eoff, n, from,
+                               to, s);
             MDET_HIP(hipGetLastError());
             result = mdet_edges{ne, from, to, s};
         }
