@@ -204,6 +204,46 @@ __device__ inline uint32_t wave_shard() {
     return ((blockIdx.x * (blockDim.x >> 6)) + (threadIdx.x >> 6)) & (NSHARD - 1);
 }
 
+// Rows of the select stage are grouped in buckets of S_BUCKET consecutive
+// positions of the prefilter's row order.
+constexpr int S_BUCKET = 128;
+
+// Adds, for each distinct key held by the active lanes, the number of lanes
+// holding it to counter[key] (one atomic by the lowest such lane) and returns
+// to every active lane old value + its rank among the lanes with its key.
+// All 64 lanes call; the loop runs once per distinct key.
+__device__ inline uint32_t wave_aggregated_add(uint32_t* counter, uint32_t key, bool active) {
+    uint32_t slot = 0;
+    uint64_t pending = __ballot(active);
+    const uint32_t lane = lane_id();
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t k = __shfl(key, leader, 64);
+        const bool mine = active && key == k;
+        const uint64_t same = __ballot(mine);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counter[k], (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if (mine) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
+
+// The same without the slots: counter[key] += lanes holding key (no return
+// value is waited for).
+__device__ inline void wave_aggregated_count(uint32_t* counter, uint32_t key, bool active) {
+    uint64_t pending = __ballot(active);
+    const uint32_t lane = lane_id();
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t k = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(active && key == k);
+        if ((int)lane == leader) atomicAdd(&counter[k], (uint32_t)__popcll(same));
+        pending &= ~same;
+    }
+}
+
 // All 64 lanes call; lane l gets the shard-local slot of its first item.
 __device__ inline unsigned long long wave_reserve_shard(uint32_t cnt, const SurvOut& o, uint32_t shard) {
     const uint32_t lane = lane_id();
@@ -707,7 +747,8 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
                                                 const uint32_t* __restrict__ perm,
                                                 const float* __restrict__ rows, const double* __restrict__ ls,
                                                 const float* __restrict__ tv, double T, Cand* __restrict__ cand,
-                                                unsigned long long* __restrict__ counters) {
+                                                unsigned long long* __restrict__ counters,
+                                                uint32_t* __restrict__ bcnt) {
     __shared__ unsigned int s_keep, s_adj;
     if (threadIdx.x == 0) s_keep = s_adj = 0;
     __syncthreads();
@@ -745,6 +786,7 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
         }
         part += __shfl_xor(part, 1, 64);
         part += __shfl_xor(part, 2, 64);
+        bool kept = false;
         if (idx < hi && q == 0) {
             const float d = sqrtf(part);
             const Pre32 m = model32(d, tv[p.x], tv[p.y]);
@@ -762,6 +804,7 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
                 key = m.pre;
             }
             nkeep += keep ? 1u : 0u;
+            kept = keep;
             Cand c;
             c.i = id.x;
             c.j = id.y;
@@ -769,6 +812,9 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
             c.flags = (amb ? 1u : 0u) | (keep ? 2u : 0u);
             cand[idx] = c;
         }
+        // rows of the select buckets: p holds prefilter row positions
+        wave_aggregated_count(bcnt, p.x / S_BUCKET, kept);
+        wave_aggregated_count(bcnt, p.y / S_BUCKET, kept);
     }
     if (nkeep) atomicAdd(&s_keep, nkeep);
     if (nadj) atomicAdd(&s_adj, nadj);
@@ -791,7 +837,6 @@ __global__ __launch_bounds__(256) void k_refine(const uint2* __restrict__ surv, 
 // bucket then groups its entries by row with LDS atomics, and one wave per
 // row selects and sorts in LDS; rows longer than W_CAP entries, and rows
 // whose FP32 tie zone at the cut exceeds one wave, take the block-wide path.
-constexpr int S_BUCKET = 256;
 constexpr int S_THREADS = 512;          // 8 waves per bucket block
 constexpr int S_WAVES = S_THREADS / 64;
 constexpr int W_CAP = 512;              // entries of a row sorted by one wave
@@ -806,49 +851,10 @@ __device__ inline uint32_t position_of(const uint32_t* __restrict__ iperm, uint3
     return iperm ? iperm[c] : c;
 }
 
-// Adds, for each distinct key held by the active lanes, the number of lanes
-// holding it to counter[key] (one atomic by the lowest such lane) and returns
-// to every active lane old value + its rank among the lanes with its key.
-// All 64 lanes call; the loop runs once per distinct key.
-__device__ inline uint32_t wave_aggregated_add(uint32_t* counter, uint32_t key, bool active) {
-    uint32_t slot = 0;
-    uint64_t pending = __ballot(active);
-    const uint32_t lane = lane_id();
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const uint32_t k = __shfl(key, leader, 64);
-        const bool mine = active && key == k;
-        const uint64_t same = __ballot(mine);
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&counter[k], (uint32_t)__popcll(same));
-        base = __shfl(base, leader, 64);
-        if (mine) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-        pending &= ~same;
-    }
-    return slot;
-}
-
 __global__ __launch_bounds__(256) void k_iperm(const uint32_t* __restrict__ perm, uint32_t n,
                                                uint32_t* __restrict__ iperm) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) iperm[perm[s]] = s;
-}
-
-__global__ __launch_bounds__(256) void k_bucket_count(const Cand* __restrict__ cand, uint64_t nc,
-                                                      const uint32_t* __restrict__ iperm, uint32_t* __restrict__ bcnt) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool act = false;
-    uint32_t bi = 0, bj = 0;
-    if (k < nc) {
-        const Cand c = cand[k];
-        if (c.flags & 2u) {
-            act = true;
-            bi = position_of(iperm, c.i) / S_BUCKET;
-            bj = position_of(iperm, c.j) / S_BUCKET;
-        }
-    }
-    (void)wave_aggregated_add(bcnt, bi, act);
-    (void)wave_aggregated_add(bcnt, bj, act);
 }
 
 __global__ __launch_bounds__(256) void k_bucket_fill(const Cand* __restrict__ cand, uint64_t nc,
@@ -1124,57 +1130,74 @@ __device__ void block_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t d
 }
 
 // One block per bucket: group the bucket's entries by row into adj (same
-// offsets as ent), then select every row.  Output per contig row: out_cnt
-// (edges kept, partners above the row, sorted) and rbeg (where they are in adj).
+// offsets as ent), then select every row by one wave.  Output per contig row:
+// out_cnt (edges kept: partners above the row, sorted) and rbeg (where they
+// are in adj).  Rows the wave path declines go to a list for
+// k_bigrow_select: big[0] = count, then (row id, degree, begin) triples.
+constexpr int BIG_REC = 4;  // uint32 words per big-row record: rid, deg, beg lo, beg hi
+
 __global__ __launch_bounds__(S_THREADS) void k_bucket_select(
     const uint64_t* __restrict__ boff, const Ent* __restrict__ ent, Adj* __restrict__ adj, uint32_t n, uint32_t K,
     const uint32_t* __restrict__ perm, const float* __restrict__ tnf, const double* __restrict__ ls,
-    uint32_t* __restrict__ out_cnt, uint64_t* __restrict__ rbeg, unsigned int* __restrict__ err,
+    uint32_t* __restrict__ out_cnt, uint64_t* __restrict__ rbeg, uint32_t* __restrict__ big, uint32_t big_cap,
     unsigned long long* __restrict__ capped) {
+    constexpr int PER = S_BUCKET / 64;
+    constexpr int BATCH = 4;
     __shared__ uint32_t hist[S_BUCKET], rstart[S_BUCKET], rcur[S_BUCKET];
     __shared__ float wkey[S_WAVES][W_CAP];
     __shared__ uint32_t wid[S_WAVES][W_CAP];
-    __shared__ float bkey[SEL_CAP];
-    __shared__ uint32_t bid[SEL_CAP];
-    __shared__ double zkey[ZONE_CAP];
-    __shared__ uint32_t big[S_BUCKET];
-    __shared__ uint32_t nbig, s_cnt;
-    __shared__ int s_z[2];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t b = blockIdx.x;
     const uint64_t e0 = boff[b], e1 = boff[b + 1];
     const uint32_t row0 = b * S_BUCKET;
     const uint32_t rows = n - row0 < (uint32_t)S_BUCKET ? n - row0 : (uint32_t)S_BUCKET;
     for (int t = tid; t < S_BUCKET; t += S_THREADS) hist[t] = rcur[t] = 0;
-    if (tid == 0) nbig = 0;
     __syncthreads();
-    for (uint64_t e = e0 + tid; e < e1; e += S_THREADS) atomicAdd(&hist[ent[e].lrow], 1u);
-    __syncthreads();
-    if (wave == 0) {  // exclusive scan of the 256 row counts, 4 per lane
-        uint32_t v[4], s = 0;
+    // row histogram; BATCH independent loads in flight per thread
+    for (uint64_t e = e0 + tid; e < e1; e += BATCH * S_THREADS) {
+        uint32_t lr[BATCH];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            v[q] = hist[4 * lane + q];
-            s += v[q];
+        for (int q = 0; q < BATCH; ++q) lr[q] = e + q * S_THREADS < e1 ? ent[e + q * S_THREADS].lrow : 0xffffffffu;
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q)
+            if (lr[q] != 0xffffffffu) atomicAdd(&hist[lr[q]], 1u);
+    }
+    __syncthreads();
+    if (wave == 0) {  // exclusive scan of the row counts, PER per lane
+        uint32_t v[PER], sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            v[q] = hist[PER * lane + q];
+            sum += v[q];
         }
-        uint32_t x = s;
+        uint32_t x = sum;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t y = __shfl_up(x, off, 64);
             if (lane >= off) x += y;
         }
-        uint32_t run = x - s;
+        uint32_t run = x - sum;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            rstart[4 * lane + q] = run;
+        for (int q = 0; q < PER; ++q) {
+            rstart[PER * lane + q] = run;
             run += v[q];
         }
     }
     __syncthreads();
-    for (uint64_t e = e0 + tid; e < e1; e += S_THREADS) {
-        const Ent x = ent[e];
-        const uint32_t s = atomicAdd(&rcur[x.lrow], 1u);
-        adj[e0 + rstart[x.lrow] + s] = Adj{x.partner, x.pre};
+    for (uint64_t e = e0 + tid; e < e1; e += BATCH * S_THREADS) {
+        Ent x[BATCH];
+        bool ok[BATCH];
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q) {
+            ok[q] = e + q * S_THREADS < e1;
+            if (ok[q]) x[q] = ent[e + q * S_THREADS];
+        }
+#pragma unroll
+        for (int q = 0; q < BATCH; ++q)
+            if (ok[q]) {
+                const uint32_t slot = atomicAdd(&rcur[x[q].lrow], 1u);
+                adj[e0 + rstart[x[q].lrow] + slot] = Adj{x[q].partner, x[q].pre};
+            }
     }
     __syncthreads();
     for (uint32_t r = wave; r < rows; r += S_WAVES) {
@@ -1193,16 +1216,39 @@ __global__ __launch_bounds__(S_THREADS) void k_bucket_select(
         } else {
             done = wave_row_select(adj, beg, deg, rid, K, tnf, ls, wkey[wave], wid[wave], out_cnt, rbeg, capped, lane);
         }
-        if (!done && lane == 0) big[atomicAdd(&nbig, 1u)] = r;
-        wave_sync();
+        if (!done && lane == 0) {
+            const uint32_t q = atomicAdd(&big[0], 1u);
+            if (q < big_cap) {
+                uint32_t* rec = big + BIG_REC * (1 + q);
+                rec[0] = rid;
+                rec[1] = deg;
+                rec[2] = (uint32_t)beg;
+                rec[3] = (uint32_t)(beg >> 32);
+            }
+        }
     }
-    __syncthreads();
-    const uint32_t nb = nbig;
-    for (uint32_t q = 0; q < nb; ++q) {
-        const uint32_t r = big[q];
-        const uint32_t pos = row0 + r;
-        block_row_select(adj, e0 + rstart[r], hist[r], perm ? perm[pos] : pos, K, tnf, ls, bkey, bid, zkey, s_z,
-                         &s_cnt, out_cnt, rbeg, err, capped);
+}
+
+// Rows the wave path declined (over W_CAP entries, or a tie zone wider than a
+// wave): one block per row, blocks striding over the list; err bit 4 if the
+// list overflowed its capacity.
+__global__ __launch_bounds__(S_THREADS) void k_bigrow_select(
+    const uint32_t* __restrict__ big, uint32_t big_cap, Adj* __restrict__ adj, uint32_t K,
+    const float* __restrict__ tnf, const double* __restrict__ ls, uint32_t* __restrict__ out_cnt,
+    uint64_t* __restrict__ rbeg, unsigned int* __restrict__ err, unsigned long long* __restrict__ capped) {
+    __shared__ float bkey[SEL_CAP];
+    __shared__ uint32_t bid[SEL_CAP];
+    __shared__ double zkey[ZONE_CAP];
+    __shared__ int s_z[2];
+    __shared__ uint32_t s_cnt;
+    const uint32_t nbig = big[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nbig > big_cap) atomicOr(err, 4u);
+    const uint32_t m = nbig < big_cap ? nbig : big_cap;
+    for (uint32_t q = blockIdx.x; q < m; q += gridDim.x) {
+        const uint32_t* rec = big + BIG_REC * (1 + q);
+        const uint64_t beg = (uint64_t)rec[2] | ((uint64_t)rec[3] << 32);
+        block_row_select(adj, beg, rec[1], rec[0], K, tnf, ls, bkey, bid, zkey, s_z, &s_cnt, out_cnt, rbeg, err,
+                         capped);
     }
 }
 
@@ -1641,9 +1687,14 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         st.survivors = ns;
         st.pairs_evaluated = st.pairs_total;
 
-        // -- refine (in place: cand[k] belongs to surv[k])
+        // -- refine (in place: cand[k] belongs to surv[k]); counts the kept
+        // candidates' endpoints per select bucket
         Cand* cand;
         MDET_TRY(ctx_buf(ctx, "pairs.cand", std::max<uint64_t>(ns, 1), &cand));
+        const uint32_t nb = (n + S_BUCKET - 1) / S_BUCKET;
+        uint32_t* bcnt;
+        MDET_TRY(ctx_buf(ctx, "sel.bcnt", (size_t)nb + 1, &bcnt));
+        MDET_HIP(hipMemsetAsync(bcnt, 0, ((size_t)nb + 1) * 4, ctx->stream));
         if (ns) {
             const float* rows = tnf_dev;
             const float* tv = tvec;
@@ -1659,7 +1710,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             uint64_t blocks = std::min<uint64_t>((ns + 63) / 64, (uint64_t)ctx->num_cus * 32);
             blocks = (blocks + 7) / 8 * 8;
             hipLaunchKernelGGL(k_refine, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, surv, ns, perm, rows,
-                               logsize_dev, tv, T, cand, rc);
+                               logsize_dev, tv, T, cand, rc, bcnt);
             MDET_HIP(hipGetLastError());
         }
         MDET_TRY(tm.record(3));
@@ -1677,14 +1728,12 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         st.adjudicated = rch[1];
 
         // -- select
-        const uint32_t nb = (n + S_BUCKET - 1) / S_BUCKET;
-        uint32_t *bcnt, *bcur, *out_cnt, *iperm = nullptr;
+        uint32_t *bcur, *out_cnt, *iperm = nullptr;
         uint64_t *boff, *eoff, *rbeg;
         Ent* ent;
         Adj* adj;
         unsigned int* err;
         unsigned long long* capped;
-        MDET_TRY(ctx_buf(ctx, "sel.bcnt", (size_t)nb + 1, &bcnt));
         MDET_TRY(ctx_buf(ctx, "sel.bcur", (size_t)nb, &bcur));
         MDET_TRY(ctx_buf(ctx, "sel.boff", (size_t)nb + 1, &boff));
         MDET_TRY(ctx_buf(ctx, "sel.outcnt", (size_t)n + 1, &out_cnt));
@@ -1694,7 +1743,10 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         MDET_TRY(ctx_buf(ctx, "sel.adj", std::max<uint64_t>(2 * st.candidates, 1), &adj));
         MDET_TRY(ctx_buf(ctx, "sel.err", 4, &err));
         capped = reinterpret_cast<unsigned long long*>(err + 2);
-        MDET_HIP(hipMemsetAsync(bcnt, 0, ((size_t)nb + 1) * 4, ctx->stream));
+        const uint32_t big_cap = std::min<uint32_t>(n, 1u << 20);
+        uint32_t* big;
+        MDET_TRY(ctx_buf(ctx, "sel.big", (size_t)BIG_REC * (big_cap + 1), &big));
+        MDET_HIP(hipMemsetAsync(big, 0, 4, ctx->stream));
         MDET_HIP(hipMemsetAsync(bcur, 0, (size_t)nb * 4, ctx->stream));
         MDET_HIP(hipMemsetAsync(out_cnt, 0, ((size_t)n + 1) * 4, ctx->stream));
         MDET_HIP(hipMemsetAsync(err, 0, 16, ctx->stream));
@@ -1704,11 +1756,12 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 hipLaunchKernelGGL(k_iperm, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, perm, n, iperm);
             }
             const unsigned cblocks = (unsigned)((nc + 255) / 256);
-            hipLaunchKernelGGL(k_bucket_count, dim3(cblocks), dim3(256), 0, ctx->stream, cand, nc, iperm, bcnt);
             MDET_TRY(exclusive_scan_u32_to_u64(ctx, bcnt, boff, nb));
             hipLaunchKernelGGL(k_bucket_fill, dim3(cblocks), dim3(256), 0, ctx->stream, cand, nc, iperm, boff, bcur, ent);
             hipLaunchKernelGGL(k_bucket_select, dim3(nb), dim3(S_THREADS), 0, ctx->stream, boff, ent, adj, n, max_edges,
-                               perm, tnf_dev, logsize_dev, out_cnt, rbeg, err, capped);
+                               perm, tnf_dev, logsize_dev, out_cnt, rbeg, big, big_cap, capped);
+            hipLaunchKernelGGL(k_bigrow_select, dim3(ctx->num_cus), dim3(S_THREADS), 0, ctx->stream, big, big_cap, adj,
+                               max_edges, tnf_dev, logsize_dev, out_cnt, rbeg, err, capped);
             MDET_HIP(hipGetLastError());
             MDET_TRY(exclusive_scan_u32_to_u64(ctx, out_cnt, eoff, n));
             unsigned long long errh[2];
@@ -1723,6 +1776,8 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: a contig has more than %d candidate partners", SEL_CAP);
             if ((unsigned int)errh[0] & 2u)
                 MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: more than %d near-tied candidates at a row's cut", ZONE_CAP);
+            if ((unsigned int)errh[0] & 4u)
+                MDET_FAIL(MDET_E_CAPACITY, "mdet_tnf_edges: more than %u rows need the block-wide selection", big_cap);
             st.rows_capped = errh[1];
             uint64_t ne;
             MDET_TRY(read_scalar(ctx, eoff + n, &ne));
