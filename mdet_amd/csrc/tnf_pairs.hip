@@ -599,6 +599,8 @@ __global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__
 // steps with survivors enter the extraction pass, whose cost scales with the
 // number of (tile, accumulator) slots that hold survivors.
 constexpr int M_ROWS = 256, M_STEP = 128, M_SEG = 2048;
+constexpr int M_MIN_BLOCKS = 2;  // resident blocks per CU the registers are sized for
+static_assert(M_STEP % 64 == 0, "steps are whole 64-column halves");
 
 template <bool BF16>
 __device__ inline f32x16_t mfma32(uint4 a, uint4 b, f32x16_t c) {
@@ -609,14 +611,18 @@ __device__ inline f32x16_t mfma32(uint4 a, uint4 b, f32x16_t c) {
                                                   0, 0);
 }
 
+// One step's column fragments (M_STEP / 32 tiles x KSX k-steps x 64 lanes of
+// 16 bytes) into LDS: each wave copies 1 KiB chunks, chunk m*4 + wave.
 template <int KSX>
 __device__ inline void issue_btile(const uint4* __restrict__ frag, uint32_t c0, uint4* lds, int tid, int wave) {
+    constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;
     const uint4* src = frag + (size_t)(c0 / 32) * KSX * 64;
 #pragma unroll
-    for (int m = 0; m < KSX; ++m)
-        __builtin_amdgcn_global_load_lds((const void*)(src + 256 * m + tid),
-                                         (__attribute__((address_space(3))) void*)(lds + 256 * m + 64 * wave), 16, 0,
-                                         0);
+    for (int m = 0; m < (STEP_U4 + 255) / 256; ++m)
+        if (STEP_U4 % 256 == 0 || 256 * m + 64 * wave < STEP_U4)
+            __builtin_amdgcn_global_load_lds((const void*)(src + 256 * m + tid),
+                                             (__attribute__((address_space(3))) void*)(lds + 256 * m + 64 * wave), 16,
+                                             0, 0);
 }
 
 // Per-wave survivor staging in LDS: survivors of a step are appended to a
@@ -638,7 +644,7 @@ __device__ inline void stage_flush(const SurvOut& o, uint32_t shard, uint2* stag
 }
 
 template <bool BF16, int KSX>
-__global__ __launch_bounds__(256, 2) void k_prefilter_mfma(const uint4* __restrict__ frag,
+__global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint4* __restrict__ frag,
                                                            const uint4* __restrict__ frag_a8,
                                                            const uint4* __restrict__ items, uint32_t n,
                                                            SurvOut out) {
@@ -675,7 +681,7 @@ __global__ __launch_bounds__(256, 2) void k_prefilter_mfma(const uint4* __restri
         // pairs with j <= i exist only where the step meets the diagonal
         const bool diag = c0 < row0 + 64;
 #pragma unroll
-        for (int hc = 0; hc < 2; ++hc) {  // two 64-column halves of the step
+        for (int hc = 0; hc < M_STEP / 64; ++hc) {  // 64-column halves of the step
             f32x16_t acc[2][2];
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
@@ -691,26 +697,31 @@ __global__ __launch_bounds__(256, 2) void k_prefilter_mfma(const uint4* __restri
 #pragma unroll
                     for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma32<BF16>(a[rt][ks], b[ct], acc[rt][ct]);
             }
-            // fast path: per-lane maxima of each 32x32 tile
-            float tmax[2][2];
+            // fast path: per-lane maxima of each 32x32 tile.  acc > 0.0f
+            // exactly when its bits, read as a signed int, are > 0, so the
+            // maxima are integer max3 trees (no NaN canonicalisation).
+            int tmax[2][2];
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
                     const f32x16_t v = acc[rt][ct];
-                    float m0 = fmaxf(fmaxf(v[0], v[1]), v[2]), m1 = fmaxf(fmaxf(v[3], v[4]), v[5]);
-                    float m2 = fmaxf(fmaxf(v[6], v[7]), v[8]), m3 = fmaxf(fmaxf(v[9], v[10]), v[11]);
-                    float m4 = fmaxf(fmaxf(v[12], v[13]), v[14]);
-                    tmax[rt][ct] = fmaxf(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)), fmaxf(m4, v[15]));
+                    int b[16];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) b[r] = __float_as_int(v[r]);
+                    const int m0 = max(max(b[0], b[1]), b[2]), m1 = max(max(b[3], b[4]), b[5]);
+                    const int m2 = max(max(b[6], b[7]), b[8]), m3 = max(max(b[9], b[10]), b[11]);
+                    const int m4 = max(max(b[12], b[13]), b[14]);
+                    tmax[rt][ct] = max(max(max(m0, m1), m2), max(max(m3, m4), b[15]));
                 }
-            const float hmax = fmaxf(fmaxf(tmax[0][0], tmax[0][1]), fmaxf(tmax[1][0], tmax[1][1]));
-            if (!__any(hmax > 0.f)) continue;
+            const int hmax = max(max(tmax[0][0], tmax[0][1]), max(tmax[1][0], tmax[1][1]));
+            if (!__any(hmax > 0)) continue;
             const uint32_t jl = c0 + 64 * hc + (lane & 31);
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
-                    if (!__any(tmax[rt][ct] > 0.f)) continue;
+                    if (!__any(tmax[rt][ct] > 0)) continue;
                     const uint32_t j = jl + 32 * ct;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
