@@ -224,13 +224,17 @@ int mdet_mlp_train_step(mdet_mlp *mlp, const float *tnf_dev, const uint32_t *pi_
 /* Embeddings emb_dev[n x 32] of TNF rows tnf_dev[n x 136]. */
 int mdet_mlp_embed(mdet_mlp *mlp, const float *tnf_dev, uint32_t n, float *emb_dev);
 /* Edge list as mdet_tnf_edges, but only pairs whose BF16 embedding distance^2
- * is below radius2 are evaluated (MFMA Gram of the 32-dimensional
- * embeddings); those are decided exactly as in the other modes.  Edges of
- * pairs the network fails to propose are missed: recall is measured against
- * mdet_tnf_edges. */
+ * is below radius2 are evaluated exactly (the same refinement and selection
+ * as the other modes).  Candidate pairs: with n_probe == 0 every pair (MFMA
+ * Gram of the 32-dimensional embeddings over the whole triangle); with
+ * n_probe > 0 (at most 16) a cluster index over the embeddings (k-means,
+ * n_clusters centroids, 0 = n/192, at most 8192) and only the pairs of
+ * clusters among each other's n_probe nearest centroids.  Edges of pairs the filter fails to
+ * propose are missed: recall is measured against mdet_tnf_edges.
+ * stats->pairs_evaluated counts the candidate pairs. */
 int mdet_tnf_edges_learned(mdet_ctx *ctx, mdet_mlp *mlp, const float *tnf_dev, const double *logsize_dev, uint32_t n,
-                           double cutoff, uint32_t max_edges, float radius2, mdet_edges *out,
-                           mdet_pair_stats *stats);
+                           double cutoff, uint32_t max_edges, float radius2, uint32_t n_clusters, uint32_t n_probe,
+                           mdet_edges *out, mdet_pair_stats *stats);
 
 /*
  * Sampled row maxima for the automatic cutoff (pTNF = 0): for each sampled

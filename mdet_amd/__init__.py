@@ -89,7 +89,8 @@ _SIGS = {
                                       C.c_float, C.c_float, C.c_void_p]),
     "mdet_mlp_embed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "mdet_tnf_edges_learned": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double,
-                                         C.c_uint32, C.c_float, C.POINTER(Edges), C.POINTER(PairStats)]),
+                                         C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Edges),
+                                         C.POINTER(PairStats)]),
     "mdet_tnf_rowmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                   C.c_void_p]),
     "mdet_tnf_preprob": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
@@ -346,12 +347,16 @@ class Ctx:
                                           C.c_void_p(tnf.ptr)), "mdet_tnf_count_ascii")
 
     def tnf_edges(self, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int,
-                  precision: int = EXACT_F32, fetch: bool = True, mlp: "Mlp | None" = None, radius2: float = 0.0):
+                  precision: int = EXACT_F32, fetch: bool = True, mlp: "Mlp | None" = None, radius2: float = 0.0,
+                  n_probe: int = 0, n_clusters: int = 0):
+        """Edge list; precision LEARNED needs mlp and radius2 (n_probe > 0: cluster index
+        with n_clusters centroids, 0 = automatic)."""
         e = Edges()
         st = PairStats()
         if precision == LEARNED:
             _check(lib().mdet_tnf_edges_learned(self.h, mlp.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n,
-                                                cutoff, max_edges, radius2, C.byref(e), C.byref(st)),
+                                                cutoff, max_edges, radius2, n_clusters, n_probe, C.byref(e),
+                                                C.byref(st)),
                    "mdet_tnf_edges_learned")
         else:
             _check(lib().mdet_tnf_edges(self.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n, cutoff, max_edges,

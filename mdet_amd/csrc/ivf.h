@@ -1,0 +1,34 @@
+// Internal interface between the cluster index of the learned filter
+// (tnf_ivf.hip) and the pair pipeline (tnf_pairs.hip).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "hip_common.h"
+
+namespace mdet {
+
+// Prefilter work item: uint4 (r_lo, r_hi, c_lo, c_hi | ITEM_TRI), the pairs of
+// row positions [r_lo, r_hi) and column positions [c_lo, c_hi) of the
+// prefilter's row order; with ITEM_TRI only pairs i < j.  A block covers the
+// ITEM_ROWS rows from r_lo rounded down to a 32-row tile.
+constexpr uint32_t ITEM_TRI = 0x80000000u;
+constexpr uint32_t ITEM_ROWS = 256;
+
+// Inverted-file candidate generation over the n x 32 embedding rows:
+// k-means clusters, rows sorted by cluster, and for every unordered pair of
+// clusters (a, b) with b among the n_probe centroids nearest to a's (or a
+// among b's), the prefilter items covering a's rows x b's rows.
+struct IvfIndex {
+    uint32_t* perm = nullptr;  // device: sorted position -> contig (context buffer)
+    std::vector<uint4> items;  // host
+    uint64_t pairs = 0;        // unordered contig pairs the items cover
+    uint32_t k = 0;            // clusters
+    uint32_t probe_pairs = 0;  // distinct unordered cluster pairs
+};
+
+// Clusters k (0: n / 192, at most 8192), probes n_probe >= 1.
+int ivf_build(mdet_ctx* ctx, const float* emb_dev, uint32_t n, uint32_t k, uint32_t n_probe, IvfIndex* out);
+
+}  // namespace mdet

@@ -185,6 +185,165 @@ __global__ void k_quantize(const float* __restrict__ p, float* __restrict__ wq, 
     wq[off + k] = round ? bf16_round(v) : v;
 }
 
+
+// ------------------------------------------------------------ inference --
+// Embeddings on MFMA (v_mfma_f32_32x32x16_bf16): BF16 weights, inputs and
+// hidden activations, FP32 accumulation.  One wave per 32 rows, 128 rows per
+// block.  Layer 1's A fragments come straight from the FP32 TNF rows
+// (standardised, NaN -> 0, rounded to BF16, K padded 136 -> 144).  A row of a
+// row-major BF16 weight matrix is an MFMA B fragment (lane l: output column
+// l % 32, inputs 8 * (l / 32) .. +8 of the k-step).  Between layers, bias +
+// GELU + BF16 rounding go through a per-wave LDS buffer of 32 rows x 256
+// (rows padded by 16 bytes against bank conflicts).  Fragment conventions as
+// in the pair prefilter: accumulator r of a lane holds row
+// 4 * (l / 32) + r % 4 + 8 * (r / 4), column l % 32.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int K1_PAD = 144;             // layer-1 inputs padded to 9 k-steps
+constexpr int H_STRIDE = D_H + 8;       // BF16 elements per LDS row
+constexpr int EMB_ROWS = 128;
+
+__device__ inline uint16_t bf16_bits(float f) { return (uint16_t)(__float_as_uint(bf16_round(f)) >> 16); }
+
+__device__ inline uint4 pack_bf16x8(const float* v) {
+    uint4 w;
+    w.x = bf16_bits(v[0]) | ((uint32_t)bf16_bits(v[1]) << 16);
+    w.y = bf16_bits(v[2]) | ((uint32_t)bf16_bits(v[3]) << 16);
+    w.z = bf16_bits(v[4]) | ((uint32_t)bf16_bits(v[5]) << 16);
+    w.w = bf16_bits(v[6]) | ((uint32_t)bf16_bits(v[7]) << 16);
+    return w;
+}
+
+__device__ inline f32x16_t mfma_bf16(uint4 a, uint4 b, f32x16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                    c, 0, 0, 0);
+}
+
+// Weights (already BF16-valued in wq) as BF16 bits, row-major [out][in]:
+// W1 [256][144] (inputs 136..143 zero), W2 [256][256], W3 [32][256].
+__global__ void k_pack_weights(const float* __restrict__ wq, uint16_t* __restrict__ w1, uint16_t* __restrict__ w2,
+                               uint16_t* __restrict__ w3) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr uint32_t N1 = D_H * K1_PAD, N2 = D_H * D_H, N3 = D_E * D_H;
+    if (e < N1) {
+        const uint32_t o = e / K1_PAD, i = e % K1_PAD;
+        w1[e] = i < (uint32_t)D_IN ? bf16_bits(wq[OFF_W1 + o * D_IN + i]) : (uint16_t)0;
+    } else if (e < N1 + N2) {
+        w2[e - N1] = bf16_bits(wq[OFF_W2 + (e - N1)]);
+    } else if (e < N1 + N2 + N3) {
+        w3[e - N1 - N2] = bf16_bits(wq[OFF_W3 + (e - N1 - N2)]);
+    }
+}
+
+// One hidden layer's output tile set (4 column tiles of 32) to LDS:
+// H[row][col] = bf16(gelu(acc + bias[col])).
+__device__ inline void store_hidden(uint16_t* H, const f32x16_t* acc, int tile0, const float* __restrict__ bias,
+                                    int lane) {
+    const int lr = lane & 31, rb = 4 * (lane >> 5);
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int col = 32 * (tile0 + ct) + lr;
+        const float bv = bias[col];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = rb + (r & 3) + 8 * (r >> 2);
+            H[row * H_STRIDE + col] = bf16_bits(gelu(acc[ct][r] + bv));
+        }
+    }
+}
+
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(256) void k_embed_mfma(const float* __restrict__ tnf, uint32_t n,
+                                                    const float* __restrict__ norm, const uint16_t* __restrict__ w1,
+                                                    const uint16_t* __restrict__ w2, const uint16_t* __restrict__ w3,
+                                                    const float* __restrict__ wq, float* __restrict__ emb) {
+    __shared__ uint16_t hbuf[4][32 * H_STRIDE];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t row0 = blockIdx.x * EMB_ROWS + wave * 32;
+    if (row0 >= n) return;  // no block-wide synchronisation below
+    const int lr = lane & 31, kh = 8 * (lane >> 5);
+    const uint32_t r = row0 + lr;
+    uint16_t* H = hbuf[wave];
+
+    // layer 1: A fragments from the TNF row
+    uint4 a[16];
+#pragma unroll
+    for (int ks = 0; ks < 9; ++ks) {
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = 16 * ks + kh + e;
+            float x = 0.f;
+            if (r < n && k < D_IN) {
+                float t = tnf[(size_t)r * D_IN + k];
+                if (t != t) t = 0.f;  // rows with no valid window
+                x = (t - norm[k]) * norm[D_IN + k];
+            }
+            v[e] = x;
+        }
+        a[ks] = pack_bf16x8(v);
+    }
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        f32x16_t acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x16_t{};
+#pragma unroll
+        for (int ks = 0; ks < 9; ++ks)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const int c = 32 * (4 * half + ct) + lr;
+                const uint4 b = *reinterpret_cast<const uint4*>(w1 + (size_t)c * K1_PAD + 16 * ks + kh);
+                acc[ct] = mfma_bf16(a[ks], b, acc[ct]);
+            }
+        store_hidden(H, acc, 4 * half, wq + OFF_B1, lane);
+    }
+    wave_lds_sync();
+
+    // layer 2: A fragments from LDS (all read before the buffer is rewritten)
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) a[ks] = *reinterpret_cast<const uint4*>(H + lr * H_STRIDE + 16 * ks + kh);
+    wave_lds_sync();
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        f32x16_t acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x16_t{};
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const int c = 32 * (4 * half + ct) + lr;
+                const uint4 b = *reinterpret_cast<const uint4*>(w2 + (size_t)c * D_H + 16 * ks + kh);
+                acc[ct] = mfma_bf16(a[ks], b, acc[ct]);
+            }
+        store_hidden(H, acc, 4 * half, wq + OFF_B2, lane);
+    }
+    wave_lds_sync();
+
+    // layer 3: 32 outputs, FP32
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) a[ks] = *reinterpret_cast<const uint4*>(H + lr * H_STRIDE + 16 * ks + kh);
+    f32x16_t acc = f32x16_t{};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        const uint4 b = *reinterpret_cast<const uint4*>(w3 + (size_t)lr * D_H + 16 * ks + kh);
+        acc = mfma_bf16(a[ks], b, acc);
+    }
+    const float bv = wq[OFF_B3 + lr];
+    const int rb = 4 * (lane >> 5);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t row = row0 + rb + (q & 3) + 8 * (q >> 2);
+        if (row < n) emb[(size_t)row * D_E + lr] = acc[q] + bv;
+    }
+}
 }  // namespace
 
 struct mdet_mlp {
@@ -437,15 +596,16 @@ int mdet_mlp_embed(mdet_mlp* m, const float* tnf_dev, uint32_t n, float* emb_dev
     MDET_TRY(check_ctx(m->ctx));
     if (n == 0) return MDET_OK;
     if (!tnf_dev || !emb_dev) MDET_FAIL(MDET_E_INVALID, "mdet_mlp_embed: null argument");
-    const int CH = 65536;  // rows per pass
-    for (uint32_t r0 = 0; r0 < n; r0 += CH) {
-        const int M = (int)std::min<uint32_t>(CH, n - r0);
-        Scratch S;
-        MDET_TRY(scratch(m, M, &S));
-        hipLaunchKernelGGL(k_gather_rows, dim3(M), dim3(160), 0, m->ctx->stream, tnf_dev + (size_t)r0 * D_IN,
-                           (uint32_t)M, m->norm, m->norm + D_IN, S.X);
-        MDET_TRY(mlp_forward(m, S.X, M, S.Z1, S.A1, S.Z2, S.A2, emb_dev + (size_t)r0 * D_E));
-    }
+    uint16_t *w1, *w2, *w3;
+    mdet_ctx* c = m->ctx;
+    MDET_TRY(ctx_buf(c, "mlp.w1bf", (size_t)D_H * K1_PAD, &w1));
+    MDET_TRY(ctx_buf(c, "mlp.w2bf", (size_t)D_H * D_H, &w2));
+    MDET_TRY(ctx_buf(c, "mlp.w3bf", (size_t)D_E * D_H, &w3));
+    const uint32_t nw = D_H * K1_PAD + D_H * D_H + D_E * D_H;
+    hipLaunchKernelGGL(k_pack_weights, dim3((nw + 255) / 256), dim3(256), 0, c->stream, m->wq, w1, w2, w3);
+    hipLaunchKernelGGL(k_embed_mfma, dim3((n + EMB_ROWS - 1) / EMB_ROWS), dim3(256), 0, c->stream, tnf_dev, n,
+                       m->norm, w1, w2, w3, m->wq, emb_dev);
+    MDET_HIP(hipGetLastError());
     return MDET_OK;
 }
 

@@ -30,11 +30,13 @@
 #include <vector>
 
 #include "hip_common.h"
+#include "ivf.h"
 #include "model_tables.h"
 #include "tnf_tables.h"
 
 namespace {
 
+using mdet::ITEM_TRI;
 constexpr int NT = 136;
 constexpr int KS = 9;    // MFMA k-steps of 16 for TNF rows (K padded to 144)
 constexpr int KS_E = 3;  // k-steps for 32-dimensional embeddings (K padded to 48)
@@ -600,6 +602,10 @@ __global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__
 // number of (tile, accumulator) slots that hold survivors.
 constexpr int M_ROWS = 256, M_STEP = 128, M_SEG = 2048;
 constexpr int M_MIN_BLOCKS = 2;  // resident blocks per CU the registers are sized for
+// Prefilter item: (r_lo, r_hi, c_lo, c_hi | ITEM_TRI) = pairs of row positions
+// [r_lo, r_hi) and column positions [c_lo, c_hi), only i < j if ITEM_TRI;
+// r_hi - (r_lo & ~31) <= M_ROWS.  The fragment arrays must hold whole 128-
+// column steps from (c_lo & ~31) up to c_hi.
 static_assert(M_STEP % 64 == 0, "steps are whole 64-column halves");
 
 template <bool BF16>
@@ -652,9 +658,14 @@ __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint
     __shared__ uint4 Bl[2][STEP_U4];
     __shared__ uint2 stage_all[4][M_STAGE];
     const uint4 it = items[blockIdx.x];
-    const uint32_t rb = it.x, c_begin = it.y, c_end = it.z;
+    const uint32_t r_lo = it.x, r_hi = it.y, c_lo = it.z, c_hi = it.w & ~ITEM_TRI;
+    const bool tri = (it.w & ITEM_TRI) != 0;
+    const uint32_t c_begin = c_lo & ~31u;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const uint32_t row0 = rb * M_ROWS + wave * 64;  // this wave's 64 rows
+    const uint32_t row0 = (r_lo & ~31u) + wave * 64;  // this wave's 64 rows
+    // a wave with no row of the item still takes its share of the column loads
+    const bool active = row0 < r_hi && row0 + 64 > r_lo;
+    const bool rows_edge = row0 < r_lo || row0 + 64 > r_hi;
     const uint32_t t0 = row0 >> 5;
     uint2* stage = stage_all[wave];
     uint32_t nstage = 0;
@@ -664,8 +675,9 @@ __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) {
 #pragma unroll
-        for (int ks = 0; ks < KSX - 1; ++ks) a[rt][ks] = frag[((size_t)(t0 + rt) * KSX + ks) * 64 + lane];
-        a[rt][KSX - 1] = frag_a8[(size_t)(t0 + rt) * 64 + lane];
+        for (int ks = 0; ks < KSX - 1; ++ks)
+            a[rt][ks] = active ? frag[((size_t)(t0 + rt) * KSX + ks) * 64 + lane] : make_uint4(0, 0, 0, 0);
+        a[rt][KSX - 1] = active ? frag_a8[(size_t)(t0 + rt) * 64 + lane] : make_uint4(0, 0, 0, 0);
     }
     // accumulator r of row tile rt holds row row0 + 32*rt + rbase + r%4 + 8*(r/4)
     const uint32_t rbase = 4 * (lane >> 5);
@@ -673,13 +685,17 @@ __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint
     const uint64_t below = (1ull << lane) - 1ull;
 
     int buf = 0;
-    for (uint32_t c0 = c_begin; c0 < c_end; c0 += M_STEP, buf ^= 1) {
+    for (uint32_t c0 = c_begin; c0 < c_hi; c0 += M_STEP, buf ^= 1) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (c0 + M_STEP < c_end) issue_btile<KSX>(frag, c0 + M_STEP, Bl[buf ^ 1], tid, wave);
+        if (c0 + M_STEP < c_hi) issue_btile<KSX>(frag, c0 + M_STEP, Bl[buf ^ 1], tid, wave);
+        if (!active) continue;
         const uint4* B = Bl[buf];
         // pairs with j <= i exist only where the step meets the diagonal
-        const bool diag = c0 < row0 + 64;
+        const bool diag = tri && c0 < row0 + 64;
+        // pairs outside the item's row or column range only where the wave
+        // or the step crosses the range's edge
+        const bool edge = rows_edge || c0 < c_lo || c0 + M_STEP > c_hi;
 #pragma unroll
         for (int hc = 0; hc < M_STEP / 64; ++hc) {  // 64-column halves of the step
             f32x16_t acc[2][2];
@@ -726,7 +742,8 @@ __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const uint32_t i = row0 + 32 * rt + rbase + (r & 3) + 8 * (r >> 2);
-                        const bool h = acc[rt][ct][r] > 0.f && (!diag || i < j);
+                        const bool h = acc[rt][ct][r] > 0.f && (!diag || i < j) &&
+                                       (!edge || (i >= r_lo && i < r_hi && j >= c_lo && j < c_hi));
                         const uint64_t m = __ballot(h);
                         if (m) {
                             const uint32_t c = (uint32_t)__popcll(m);
@@ -1507,7 +1524,7 @@ int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nite
     const uint32_t nrb = n_pad / M_ROWS;
     for (uint32_t rb = 0; rb < nrb; ++rb)
         for (uint32_t c = rb * M_ROWS; c < n_pad; c += M_SEG)
-            items.push_back(make_uint4(rb, c, std::min<uint32_t>(c + M_SEG, n_pad), 0));
+            items.push_back(make_uint4(rb * M_ROWS, rb * M_ROWS + M_ROWS, c, std::min<uint32_t>(c + M_SEG, n_pad) | ITEM_TRI));
     *nitems = (uint32_t)items.size();
     MDET_TRY(ctx_buf(ctx, "pairs.items", items.size(), items_dev));
     MDET_HIP(hipMemcpyAsync(*items_dev, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice,
@@ -1527,8 +1544,8 @@ extern "C" {
 namespace {
 
 int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
-               uint32_t max_edges, int precision, mdet_mlp* mlp, float radius2, mdet_edges* out,
-               mdet_pair_stats* stats) {
+               uint32_t max_edges, int precision, mdet_mlp* mlp, float radius2, uint32_t n_clusters,
+               uint32_t n_probe, mdet_edges* out, mdet_pair_stats* stats) {
     MDET_TRY(check_ctx(ctx));
     if (!out) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: null output");
     ctx->last_pairs = 0;
@@ -1611,6 +1628,9 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         uint2 *sbuf = nullptr, *surv = nullptr;
         uint32_t* perm = nullptr;  // sorted position -> contig id (MFMA modes)
         uint64_t ns = 0;
+        IvfIndex ivf;  // learned mode with probes
+        uint4* ivf_items = nullptr;
+        uint32_t ivf_nitems = 0;
         for (int attempt = 0; attempt < 2; ++attempt) {
             MDET_TRY(ctx_buf(ctx, "pairs.sbuf", (size_t)NSHARD * shard_cap, &sbuf));
             MDET_HIP(hipMemsetAsync(scnt, 0, (size_t)NSHARD * SHARD_STRIDE * 8, ctx->stream));
@@ -1628,22 +1648,40 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 hipLaunchKernelGGL(k_prefilter_f32, dim3((unsigned)ntri), dim3(256), 0, ctx->stream, tnft, n, n_pad,
                                    ntiles, R2, so);
             } else if (precision == MDET_LEARNED) {
-                // embedding rows (n x 32) in BF16, threshold = calibrated radius^2
+                // embedding rows (n x 32) in BF16, threshold = calibrated radius^2;
+                // with probes, only the pairs of probed cluster pairs (items of
+                // the cluster index), else every pair in composition order
                 float* emb;
                 MDET_TRY(ctx_buf(ctx, "learn.emb", (size_t)n * 32, &emb));
-                if (attempt == 0) MDET_TRY(mdet_mlp_embed(mlp, tnf_dev, n, emb));
+                if (attempt == 0) {
+                    MDET_TRY(mdet_mlp_embed(mlp, tnf_dev, n, emb));
+                    if (n_probe) {
+                        MDET_TRY(ivf_build(ctx, emb, n, n_clusters, n_probe, &ivf));
+                        ivf_nitems = (uint32_t)ivf.items.size();
+                        MDET_TRY(ctx_buf(ctx, "ivf.items", std::max<size_t>(ivf.items.size(), 1), &ivf_items));
+                        if (ivf_nitems)
+                            MDET_HIP(hipMemcpyAsync(ivf_items, ivf.items.data(), ivf.items.size() * sizeof(uint4),
+                                                    hipMemcpyHostToDevice, ctx->stream));
+                    }
+                }
+                // cluster items start column steps anywhere: room for one more block
+                const uint32_t n_frag = n_probe ? n_pad + ITEM_ROWS : n_pad;
                 uint4 *frag, *frag_a;
-                MDET_TRY(ctx_buf(ctx, "learn.frag", (size_t)n_pad / 32 * KS_E * 64, &frag));
-                MDET_TRY(ctx_buf(ctx, "learn.frag_a", (size_t)n_pad / 32 * 64, &frag_a));
-                MDET_TRY(sort_by_composition(ctx, tnf_dev, n, &perm));
-                hipLaunchKernelGGL((k_prep_lowp<true, 32, KS_E>), dim3(n_pad / 4), dim3(256), 0, ctx->stream, emb,
-                                   perm, n, n_pad, radius2, frag, frag_a);
-                uint4* items;
-                uint32_t nitems;
-                MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                MDET_TRY(ctx_buf(ctx, "learn.frag", (size_t)n_frag / 32 * KS_E * 64, &frag));
+                MDET_TRY(ctx_buf(ctx, "learn.frag_a", (size_t)n_frag / 32 * 64, &frag_a));
+                if (n_probe)
+                    perm = ivf.perm;
+                else
+                    MDET_TRY(sort_by_composition(ctx, tnf_dev, n, &perm));
+                hipLaunchKernelGGL((k_prep_lowp<true, 32, KS_E>), dim3(n_frag / 4), dim3(256), 0, ctx->stream, emb,
+                                   perm, n, n_frag, radius2, frag, frag_a);
+                uint4* items = ivf_items;
+                uint32_t nitems = ivf_nitems;
+                if (!n_probe) MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
                 if (attempt == 0) MDET_TRY(tm.record(1));
-                hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), dim3(nitems), dim3(256), 0, ctx->stream, frag, frag_a,
-                                   items, n, so);
+                if (nitems)
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
+                                       frag_a, items, n, so);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
                 // |d~ - d| <= |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit
@@ -1696,7 +1734,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                            soff, surv);
         MDET_HIP(hipGetLastError());
         st.survivors = ns;
-        st.pairs_evaluated = st.pairs_total;
+        st.pairs_evaluated = (precision == MDET_LEARNED && n_probe) ? ivf.pairs : st.pairs_total;
 
         // -- refine (in place: cand[k] belongs to surv[k]); counts the kept
         // candidates' endpoints per select bucket
@@ -1826,12 +1864,14 @@ int mdet_tnf_edges(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_de
                    uint32_t max_edges, int precision, mdet_edges* out, mdet_pair_stats* stats) {
     if (precision == MDET_LEARNED)
         MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: the learned mode needs mdet_tnf_edges_learned");
-    return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, precision, nullptr, 0.f, out, stats);
+    return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, precision, nullptr, 0.f, 0, 0, out, stats);
 }
 
 int mdet_tnf_edges_learned(mdet_ctx* ctx, mdet_mlp* mlp, const float* tnf_dev, const double* logsize_dev, uint32_t n,
-                           double cutoff, uint32_t max_edges, float radius2, mdet_edges* out, mdet_pair_stats* stats) {
-    return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, MDET_LEARNED, mlp, radius2, out, stats);
+                           double cutoff, uint32_t max_edges, float radius2, uint32_t n_clusters, uint32_t n_probe,
+                           mdet_edges* out, mdet_pair_stats* stats) {
+    return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, MDET_LEARNED, mlp, radius2, n_clusters, n_probe,
+                      out, stats);
 }
 
 int mdet_last_pairs(mdet_ctx* ctx, const void** records_dev, uint64_t* count) {

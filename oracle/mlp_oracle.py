@@ -58,11 +58,14 @@ def standardise(tnf: np.ndarray, mean: np.ndarray, istd: np.ndarray) -> np.ndarr
     return (x - mean.astype(np.float64)) * istd.astype(np.float64)
 
 
-def embed(p, mean, istd, tnf, round_weights=True):
+def embed(p, mean, istd, tnf, round_weights=True, round_acts=False):
+    """Embeddings; round_acts also rounds the standardised inputs and both
+    hidden activations to BF16 (the inference kernel's arithmetic)."""
     w = unpack(p, round_weights)
-    x = standardise(tnf, mean, istd)
-    h1 = gelu(x @ w["W1"].T + w["b1"])
-    h2 = gelu(h1 @ w["W2"].T + w["b2"])
+    act = (lambda a: bf16_round(a.astype(np.float32)).astype(np.float64)) if round_acts else (lambda a: a)
+    x = act(standardise(tnf, mean, istd))
+    h1 = act(gelu(x @ w["W1"].T + w["b1"]))
+    h2 = act(gelu(h1 @ w["W2"].T + w["b2"]))
     return h2 @ w["W3"].T + w["b3"]
 
 
