@@ -978,6 +978,77 @@ __device__ inline int pow2_at_least(uint32_t v) {
     return sz;
 }
 
+// Bitonic sort of 64 * E elements held in registers by one wave, element
+// t = e * 64 + lane in slot e of lane t % 64 (ascending by ranks_before if
+// BY_RANK, else by id; the ids of a row are distinct).  Strides below 64 are
+// exchanges between lanes (shuffles), strides of 64 and more swap slots
+// within a lane.
+template <int E, bool BY_RANK>
+__device__ inline bool elem_before(float ka, uint32_t ia, float kb, uint32_t ib) {
+    return BY_RANK ? ranks_before(ka, ia, kb, ib) : ia < ib;
+}
+
+template <int E, bool BY_RANK>
+__device__ void wave_sort_regs(float (&key)[E], uint32_t (&id)[E], int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64 * E; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int sj = j / 64;
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    if ((e & sj) == 0) {
+                        const int e2 = e | sj;
+                        const bool up = ((e * 64 + lane) & k) == 0;
+                        if (elem_before<E, BY_RANK>(key[e2], id[e2], key[e], id[e]) == up) {
+                            const float tk = key[e];
+                            key[e] = key[e2];
+                            key[e2] = tk;
+                            const uint32_t ti = id[e];
+                            id[e] = id[e2];
+                            id[e2] = ti;
+                        }
+                    }
+            } else {
+                const bool lower = (lane & j) == 0;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const float pk = __shfl_xor(key[e], j, 64);
+                    const uint32_t pi = __shfl_xor(id[e], j, 64);
+                    const bool up = ((e * 64 + lane) & k) == 0;
+                    const bool mine_first = elem_before<E, BY_RANK>(key[e], id[e], pk, pi);
+                    if ((lower == up) != mine_first) {
+                        key[e] = pk;
+                        id[e] = pi;
+                    }
+                }
+            }
+        }
+}
+
+// cnt (<= 64 * E) compacted entries of LDS key/id: sort by id in registers and
+// write them to adj[beg..].
+template <int E>
+__device__ void sort_ids_and_store(const float* key, const uint32_t* id, uint32_t cnt, Adj* __restrict__ adj,
+                                   uint64_t beg, int lane) {
+    float k[E];
+    uint32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t t = e * 64 + lane;
+        const bool in = t < cnt;
+        k[e] = in ? key[t] : 0.f;
+        v[e] = in ? id[t] : 0xffffffffu;
+    }
+    wave_sort_regs<E, false>(k, v, lane);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t t = e * 64 + lane;
+        if (t < cnt) adj[beg + t] = Adj{v[e], k[e]};
+    }
+}
+
 // Row `rid` (contig id), deg entries at adj[beg..): by one wave.  Returns
 // false, touching nothing in global memory, if the row needs the block path
 // (FP32 tie zone at the cut wider than a wave).
@@ -1038,7 +1109,8 @@ __device__ bool wave_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t de
             if (t >= (int)K || id[t] <= rid) id[t] = 0xffffffffu;
         wave_sync();
     } else {
-        // every candidate is selected: gather the partners above the row
+        // every candidate is selected: gather the partners above the row,
+        // then sort them by id in registers
         uint32_t cnt = 0;
         for (uint32_t t0 = 0; t0 < deg; t0 += 64) {
             const uint32_t t = t0 + lane;
@@ -1053,9 +1125,20 @@ __device__ bool wave_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t de
             }
             cnt += (uint32_t)__popcll(m);
         }
-        sz = pow2_at_least(cnt);
-        for (int t = (int)cnt + lane; t < sz; t += 64) id[t] = 0xffffffffu;
         wave_sync();
+        if (cnt <= 64)
+            sort_ids_and_store<1>(key, id, cnt, adj, beg, lane);
+        else if (cnt <= 128)
+            sort_ids_and_store<2>(key, id, cnt, adj, beg, lane);
+        else if (cnt <= 256)
+            sort_ids_and_store<4>(key, id, cnt, adj, beg, lane);
+        else
+            sort_ids_and_store<8>(key, id, cnt, adj, beg, lane);
+        if (lane == 0) {
+            out_cnt[rid] = cnt;
+            rbeg[rid] = beg;
+        }
+        return true;
     }
     wave_sort<false>(key, id, sz, lane);
     uint32_t cnt = 0;
