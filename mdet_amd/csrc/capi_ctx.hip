@@ -32,6 +32,19 @@ int ctx_buf(mdet_ctx* ctx, const char* key, size_t bytes, void** out) {
     return MDET_OK;
 }
 
+int ctx_host_buf(mdet_ctx* ctx, const char* key, size_t bytes, void** out) {
+    auto& b = ctx->hbufs[key];
+    if (b.bytes < bytes) {
+        if (b.p) MDET_HIP(hipHostFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+        MDET_HIP(hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
+        b.bytes = bytes;
+    }
+    *out = b.p;
+    return MDET_OK;
+}
+
 int ctx_host_stage(mdet_ctx* ctx, size_t bytes, void** out) {
     if (ctx->host_stage_bytes < bytes) {
         if (ctx->host_stage) MDET_HIP(hipHostFree(ctx->host_stage));
@@ -91,6 +104,8 @@ int mdet_ctx_destroy(mdet_ctx* ctx) {
         if (kv.second.p) (void)hipFree(kv.second.p);
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+    for (auto& kv : ctx->hbufs)
+        if (kv.second.p) (void)hipHostFree(kv.second.p);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MDET_OK;

@@ -33,7 +33,8 @@ struct mdet_ctx {
         void* p = nullptr;
         size_t bytes = 0;
     };
-    std::unordered_map<std::string, Buf> bufs;  // scratch, grown on demand
+    std::unordered_map<std::string, Buf> bufs;   // scratch, grown on demand
+    std::unordered_map<std::string, Buf> hbufs;  // pinned host buffers, grown on demand
     std::vector<hipEvent_t> events;
     // pinned host staging for small device->host reads (counters)
     void* host_stage = nullptr;
@@ -63,6 +64,8 @@ int ctx_buf(mdet_ctx* ctx, const char* key, size_t count, T** out) {
     return r;
 }
 int ctx_host_stage(mdet_ctx* ctx, size_t bytes, void** out);
+// Pinned host buffer `key` of at least `bytes` bytes, owned by the context.
+int ctx_host_buf(mdet_ctx* ctx, const char* key, size_t bytes, void** out);
 int ctx_event(mdet_ctx* ctx, int idx, hipEvent_t* out);
 int check_ctx(mdet_ctx* ctx);
 

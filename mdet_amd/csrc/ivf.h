@@ -21,14 +21,23 @@ constexpr uint32_t ITEM_ROWS = 256;
 // clusters (a, b) with b among the n_probe centroids nearest to a's (or a
 // among b's), the prefilter items covering a's rows x b's rows.
 struct IvfIndex {
-    uint32_t* perm = nullptr;  // device: sorted position -> contig (context buffer)
-    std::vector<uint4> items;  // host
-    uint64_t pairs = 0;        // unordered contig pairs the items cover
-    uint32_t k = 0;            // clusters
-    uint32_t probe_pairs = 0;  // distinct unordered cluster pairs
+    uint32_t* perm = nullptr;    // device: sorted position -> contig (context buffer)
+    uint4* items_dev = nullptr;  // device: prefilter items (context buffer)
+    uint32_t nitems = 0;
+    uint64_t pairs = 0;          // unordered contig pairs the items cover
+    uint32_t k = 0;              // clusters
+    uint32_t n_probe = 0;
+    uint32_t probe_pairs = 0;    // distinct unordered cluster pairs
+    const uint32_t* h_off = nullptr;     // pinned host copies, valid after ivf_finish's wait
+    const uint32_t* h_probes = nullptr;
 };
 
-// Clusters k (0: n / 192, at most 8192), probes n_probe >= 1.
-int ivf_build(mdet_ctx* ctx, const float* emb_dev, uint32_t n, uint32_t k, uint32_t n_probe, IvfIndex* out);
+// Device part: clusters k (0: n / 192, at most 8192), probes 1 <= n_probe <=
+// 16; sets perm and queues the copies of the cluster offsets and probe lists
+// to pinned host memory.  Work queued after it overlaps ivf_finish's host part.
+int ivf_launch(mdet_ctx* ctx, const float* emb_dev, uint32_t n, uint32_t k, uint32_t n_probe, IvfIndex* out);
+// Host part: waits for those copies only, builds the items and queues their
+// upload (stream-ordered before later launches).
+int ivf_finish(mdet_ctx* ctx, IvfIndex* out);
 
 }  // namespace mdet

@@ -50,15 +50,16 @@ __global__ void k_km_norms(const float* __restrict__ cent, uint32_t k, float* __
 }
 
 // Nearest centroid (smallest |c|^2 - 2 x.c, lowest index on ties) of the
-// points q * stride, q < m.  Block = 64 points x 4 waves: lane l of every wave
-// holds point l's 32 coordinates in registers, wave w scans the centroids
-// c = w, w + 4, ... (their coordinates are wave-uniform: scalar loads, FMAs
-// from SGPRs; two centroids per iteration on independent accumulator chains),
-// and the four partial minima are merged in LDS.  mode 1: counts[best] += 1;
+// points q * stride, q < m.  Block = 64 points x KM_WAVES waves: lane l of
+// every wave holds point l's 32 coordinates in registers, wave w scans the
+// centroids c = w, w + KM_WAVES, ... (their coordinates are wave-uniform:
+// scalar loads, FMAs from SGPRs; two centroids per iteration on independent
+// accumulator chains), and the partial minima are merged in LDS.  More waves
+// per point for the small sampled passes, whose latency would otherwise be
+// exposed.  mode 1: counts[best] += 1;
 // mode 2: also the point's coordinates into the fixed-point sums of its
 // centroid.
-constexpr int KM_WAVES = 4;
-
+template <int KM_WAVES>
 __global__ __launch_bounds__(64 * KM_WAVES) void k_km_assign(const float* __restrict__ emb, uint32_t m,
                                                             uint32_t stride, const float* __restrict__ cent,
                                                             const float* __restrict__ cnorm, uint32_t k,
@@ -261,7 +262,9 @@ __global__ void k_iota(uint32_t* __restrict__ v, uint32_t n) {
 
 namespace mdet {
 
-int ivf_build(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t n_probe, IvfIndex* out) {
+constexpr int IVF_EVENT = 8;  // context event index marking the host copies
+
+int ivf_launch(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t n_probe, IvfIndex* out) {
     if (!out || !emb || n == 0 || n_probe == 0) MDET_FAIL(MDET_E_INVALID, "ivf_build: bad arguments");
     if (k == 0) k = std::max<uint32_t>(1, (n + 96) / 192);
     k = std::min<uint32_t>(std::min<uint32_t>(k, KM_MAX_K), n);
@@ -291,11 +294,11 @@ int ivf_build(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t 
     const uint32_t stride = std::max<uint32_t>(1, n / std::max<uint32_t>(1, k * KM_SAMPLE_PER_CLUSTER));
     const uint32_t m = (n - 1) / stride + 1;
     for (int it = 0; it < KM_ITERS; ++it) {
-        hipLaunchKernelGGL(k_km_assign, dim3((m + 63) / 64), dim3(64 * KM_WAVES), 0, s, emb, m, stride, cent, cnorm, k,
+        hipLaunchKernelGGL(k_km_assign<16>, dim3((m + 63) / 64), dim3(64 * 16), 0, s, emb, m, stride, cent, cnorm, k,
                            (uint32_t*)nullptr, counts, sums, 2);
         hipLaunchKernelGGL(k_km_update, dim3((k + 7) / 8), dim3(256), 0, s, sums, counts, k, cent, cnorm);
     }
-    hipLaunchKernelGGL(k_km_assign, dim3((n + 63) / 64), dim3(64 * KM_WAVES), 0, s, emb, n, 1u, cent, cnorm, k, assign,
+    hipLaunchKernelGGL(k_km_assign<4>, dim3((n + 63) / 64), dim3(64 * 4), 0, s, emb, n, 1u, cent, cnorm, k, assign,
                        counts, (unsigned long long*)nullptr, 1);
     MDET_HIP(hipGetLastError());
 
@@ -318,11 +321,31 @@ int ivf_build(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t 
     hipLaunchKernelGGL(k_km_probe, dim3((k + 3) / 4), dim3(256), 0, s, cent, counts, k, n_probe, probes);
     MDET_HIP(hipGetLastError());
 
-    // 4. items on the host
-    std::vector<uint32_t> offh(k + 1), probeh((size_t)k * n_probe);
-    MDET_HIP(hipMemcpyAsync(offh.data(), off, offh.size() * 4, hipMemcpyDeviceToHost, s));
-    MDET_HIP(hipMemcpyAsync(probeh.data(), probes, probeh.size() * 4, hipMemcpyDeviceToHost, s));
-    MDET_HIP(hipStreamSynchronize(s));
+    // 4. copies for the host part
+    void* hb;
+    MDET_TRY(ctx_host_buf(ctx, "ivf.h_index", ((size_t)k + 1 + (size_t)k * n_probe) * 4, &hb));
+    uint32_t* h_off = static_cast<uint32_t*>(hb);
+    uint32_t* h_probes = h_off + k + 1;
+    MDET_HIP(hipMemcpyAsync(h_off, off, ((size_t)k + 1) * 4, hipMemcpyDeviceToHost, s));
+    MDET_HIP(hipMemcpyAsync(h_probes, probes, (size_t)k * n_probe * 4, hipMemcpyDeviceToHost, s));
+    hipEvent_t ev;
+    MDET_TRY(ctx_event(ctx, IVF_EVENT, &ev));
+    MDET_HIP(hipEventRecord(ev, s));
+    out->h_off = h_off;
+    out->h_probes = h_probes;
+    out->n_probe = n_probe;
+    out->perm = perm;
+    out->k = k;
+    return MDET_OK;
+}
+
+int ivf_finish(mdet_ctx* ctx, IvfIndex* out) {
+    hipEvent_t ev;
+    MDET_TRY(ctx_event(ctx, IVF_EVENT, &ev));
+    MDET_HIP(hipEventSynchronize(ev));
+    const uint32_t k = out->k, n_probe = out->n_probe;
+    const uint32_t* offh = out->h_off;
+    const uint32_t* probeh = out->h_probes;
     std::vector<uint64_t> cp;
     cp.reserve((size_t)k * n_probe);
     for (uint32_t a = 0; a < k; ++a) {
@@ -335,28 +358,37 @@ int ivf_build(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t 
     }
     std::sort(cp.begin(), cp.end());
     cp.erase(std::unique(cp.begin(), cp.end()), cp.end());
-    out->items.clear();
-    out->pairs = 0;
+    std::vector<uint4> items;
+    items.reserve(cp.size() * 2);
+    uint64_t pairs = 0;
     for (uint64_t key : cp) {
         const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
         const uint32_t a0 = offh[a], a1 = offh[a + 1], b0 = offh[b], b1 = offh[b + 1];
         for (uint32_t rs = a0; rs < a1;) {
             const uint32_t re = std::min<uint32_t>((rs & ~31u) + ITEM_ROWS, a1);
             if (a == b) {
-                if (rs + 1 < a1) out->items.push_back(make_uint4(rs, re, rs, a1 | ITEM_TRI));
+                if (rs + 1 < a1) items.push_back(make_uint4(rs, re, rs, a1 | ITEM_TRI));
                 // pairs i < j, i in [rs, re), j in [rs, a1): sum of (a1 - 1 - i)
                 const uint64_t cnt = re - rs;
-                out->pairs += cnt * (a1 - 1) - (cnt * (2 * (uint64_t)rs + cnt - 1)) / 2;
+                pairs += cnt * (a1 - 1) - (cnt * (2 * (uint64_t)rs + cnt - 1)) / 2;
             } else {
-                out->items.push_back(make_uint4(rs, re, b0, b1));
-                out->pairs += (uint64_t)(re - rs) * (b1 - b0);
+                items.push_back(make_uint4(rs, re, b0, b1));
+                pairs += (uint64_t)(re - rs) * (b1 - b0);
             }
             rs = re;
         }
     }
-    out->perm = perm;
-    out->k = k;
+    out->pairs = pairs;
     out->probe_pairs = (uint32_t)cp.size();
+    out->nitems = (uint32_t)items.size();
+    MDET_TRY(ctx_buf(ctx, "ivf.items", std::max<size_t>(items.size(), 1), &out->items_dev));
+    if (!items.empty()) {
+        // the previous call's upload from this buffer finished before that call returned
+        void* hb;
+        MDET_TRY(ctx_host_buf(ctx, "ivf.h_items", items.size() * sizeof(uint4), &hb));
+        memcpy(hb, items.data(), items.size() * sizeof(uint4));
+        MDET_HIP(hipMemcpyAsync(out->items_dev, hb, items.size() * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
+    }
     return MDET_OK;
 }
 

@@ -1712,8 +1712,18 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         uint32_t* perm = nullptr;  // sorted position -> contig id (MFMA modes)
         uint64_t ns = 0;
         IvfIndex ivf;  // learned mode with probes
-        uint4* ivf_items = nullptr;
-        uint32_t ivf_nitems = 0;
+        // FP32 rows in the prefilter's order for refinement (MFMA modes)
+        float *tnfs = nullptr, *tvs = nullptr;
+        bool rows_sorted = false;
+        auto sort_rows = [&]() -> int {
+            if (rows_sorted) return MDET_OK;
+            MDET_TRY(ctx_buf(ctx, "pairs.tnfs", (size_t)n_pad * NT, &tnfs));
+            MDET_TRY(ctx_buf(ctx, "pairs.tvs", (size_t)n_pad, &tvs));
+            hipLaunchKernelGGL(k_sort_rows, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, tvec, perm, n, n_pad,
+                               tnfs, tvs);
+            rows_sorted = true;
+            return MDET_OK;
+        };
         for (int attempt = 0; attempt < 2; ++attempt) {
             MDET_TRY(ctx_buf(ctx, "pairs.sbuf", (size_t)NSHARD * shard_cap, &sbuf));
             MDET_HIP(hipMemsetAsync(scnt, 0, (size_t)NSHARD * SHARD_STRIDE * 8, ctx->stream));
@@ -1738,14 +1748,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 MDET_TRY(ctx_buf(ctx, "learn.emb", (size_t)n * 32, &emb));
                 if (attempt == 0) {
                     MDET_TRY(mdet_mlp_embed(mlp, tnf_dev, n, emb));
-                    if (n_probe) {
-                        MDET_TRY(ivf_build(ctx, emb, n, n_clusters, n_probe, &ivf));
-                        ivf_nitems = (uint32_t)ivf.items.size();
-                        MDET_TRY(ctx_buf(ctx, "ivf.items", std::max<size_t>(ivf.items.size(), 1), &ivf_items));
-                        if (ivf_nitems)
-                            MDET_HIP(hipMemcpyAsync(ivf_items, ivf.items.data(), ivf.items.size() * sizeof(uint4),
-                                                    hipMemcpyHostToDevice, ctx->stream));
-                    }
+                    if (n_probe) MDET_TRY(ivf_launch(ctx, emb, n, n_clusters, n_probe, &ivf));
                 }
                 // cluster items start column steps anywhere: room for one more block
                 const uint32_t n_frag = n_probe ? n_pad + ITEM_ROWS : n_pad;
@@ -1758,8 +1761,12 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                     MDET_TRY(sort_by_composition(ctx, tnf_dev, n, &perm));
                 hipLaunchKernelGGL((k_prep_lowp<true, 32, KS_E>), dim3(n_frag / 4), dim3(256), 0, ctx->stream, emb,
                                    perm, n, n_frag, radius2, frag, frag_a);
-                uint4* items = ivf_items;
-                uint32_t nitems = ivf_nitems;
+                if (n_probe && attempt == 0) {
+                    MDET_TRY(sort_rows());         // device work queued behind the index copies ...
+                    MDET_TRY(ivf_finish(ctx, &ivf));  // ... runs while the host builds the items
+                }
+                uint4* items = ivf.items_dev;
+                uint32_t nitems = ivf.nitems;
                 if (!n_probe) MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (nitems)
@@ -1831,11 +1838,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             const float* rows = tnf_dev;
             const float* tv = tvec;
             if (perm) {  // MFMA modes: survivors are sorted positions
-                float *tnfs, *tvs;
-                MDET_TRY(ctx_buf(ctx, "pairs.tnfs", (size_t)n_pad * NT, &tnfs));
-                MDET_TRY(ctx_buf(ctx, "pairs.tvs", (size_t)n_pad, &tvs));
-                hipLaunchKernelGGL(k_sort_rows, dim3(n_pad / 4), dim3(256), 0, ctx->stream, tnf_dev, tvec, perm, n, n_pad,
-                                   tnfs, tvs);
+                MDET_TRY(sort_rows());
                 rows = tnfs;
                 tv = tvs;
             }
