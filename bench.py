@@ -135,9 +135,10 @@ def cpu_count_baseline(w, cfg_name, budget_contigs=30000):
                       f"openmp/TNF.cpp counts), its own timer around the TNF loop: {secs:.3f} s"}
 
 
-def run_learned(ctx, w, fb, tb, train_steps, steps, quantile=0.9995):
+def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9995):
     """Train the filter on pairs the exact path labels, then time the learned
-    edge list and measure its recall against the exhaustive edge set."""
+    edge list (cluster index over the embeddings, n_probe nearest clusters)
+    and measure its recall against the exhaustive edge set."""
     n = w.n
     t0 = time.perf_counter()
     pi, pj, y, pos = L.training_set(ctx, w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, n_pairs=10_000_000, seed=1)
@@ -149,17 +150,18 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, quantile=0.9995):
                              log_every=max(1, train_steps // 10))
     emb = L.embeddings(ctx, mlp, w.tnf, n)
     r2 = L.calibrate_radius2(emb, pos["i"], pos["j"], quantile=quantile)
-    for _ in range(1):
-        ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, mlp=mlp, radius2=r2)
+    kw = dict(mlp=mlp, radius2=r2, n_probe=n_probe)
+    for _ in range(2):
+        ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, **kw)
     ctx.sync()
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        _, st = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, mlp=mlp, radius2=r2)
+        _, st = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, **kw)
         stats.append(st.as_dict())
     ctx.sync()
     sec = (time.perf_counter() - t0) / steps
-    f, t, s, st = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, mlp=mlp, radius2=r2)
+    f, t, s, st = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, **kw)
     exact = fb.astype(np.uint64) * n + tb.astype(np.uint64)
     got = f.astype(np.uint64) * n + t.astype(np.uint64)
     hit = np.isin(exact, got)
@@ -167,6 +169,7 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, quantile=0.9995):
     return {"pairs_per_s": n * (n - 1) // 2 / sec, "ms_per_step": sec * 1e3,
             "stage_ms": {k: float(np.mean([x[k] for x in stats]))
                          for k in ("ms_prefilter", "ms_refine", "ms_select", "ms_total")},
+            "candidate_pairs": stats[-1]["pairs_evaluated"], "n_probe": n_probe,
             "survivors_exact_evaluated": stats[-1]["survivors"], "edges": int(len(f)),
             "edge_recall": float(hit.mean()) if len(exact) else 1.0,
             "edges_not_in_exhaustive_set": int((~np.isin(got, exact)).sum()),
@@ -234,6 +237,7 @@ def main():
     ap.add_argument("--modes", default="exact_f32,f16_refine")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--n-probe", type=int, default=16, help="clusters probed per cluster by the learned filter")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain copy of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -288,7 +292,7 @@ def main():
         modes["bf16_refine"]["speedup_vs_exact_f32"] = modes["bf16_refine"]["pairs_per_s"] / modes["exact_f32"]["pairs_per_s"]
     if not args.no_learned:
         log(f"learned filter: {args.train_steps} training steps")
-        lm = run_learned(ctx, w, fb, tb, args.train_steps, max(1, args.steps // 3))
+        lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, args.n_probe)
         lm["speedup_vs_bf16_refine"] = lm["pairs_per_s"] / modes["bf16_refine"]["pairs_per_s"]
         modes["learned"] = lm
 

@@ -7,7 +7,7 @@
 // most partner pairs in the same cluster, and almost all of the rest in one of
 // the few clusters whose centroids are nearest.  So the learned path evaluates
 // its embedding Gram only on the pairs of probed cluster pairs:
-//   1. k-means (Lloyd) on a sample of the rows, 8 iterations; centroid sums
+//   1. k-means (Lloyd) on a sample of the rows, 4 iterations; centroid sums
 //      in 2^-20 fixed point (integer atomics: the result does not depend on
 //      the order of the additions), then one assignment of every row;
 //   2. rows sorted by cluster (radix sort, stable), cluster offsets;
@@ -29,8 +29,11 @@ namespace {
 constexpr int E_D = 32;                 // embedding dimensions
 constexpr uint32_t KM_MAX_K = 8192;
 constexpr double KM_FIX = 1048576.0;    // fixed-point scale of the centroid sums
-constexpr int KM_ITERS = 8;
-constexpr uint32_t KM_SAMPLE_PER_CLUSTER = 32;
+// Lloyd iterations on a sample of 16 rows per centroid: on trained 200k-contig
+// embeddings 4 x 16 keeps as many partner pairs within 8 probes as 8 x 32
+// (measured), at a quarter of the cost; 2 iterations lose recall.
+constexpr int KM_ITERS = 4;
+constexpr uint32_t KM_SAMPLE_PER_CLUSTER = 16;
 
 // centroid c <- row c * n / k (rows are in input order)
 __global__ void k_km_init(const float* __restrict__ emb, uint32_t n, uint32_t k, float* __restrict__ cent) {
