@@ -72,33 +72,42 @@ __device__ inline int64_t first_start_ge(const uint64_t* runs, int64_t nr, uint6
     return lo;
 }
 
+// Per chunk: the contigs holding its first and last base and the N runs that
+// can touch its windows (binary searches, one thread per chunk, so the
+// counting blocks do not start with a chain of dependent loads).
+// meta = (first contig, last contig, first run, end run); contig -1 -> 0.
+__global__ void k_chunk_meta(uint64_t n_bases, const uint64_t* __restrict__ runs, uint64_t n_runs,
+                             const uint64_t* __restrict__ base_off, uint32_t n, uint64_t nchunks,
+                             longlong4* __restrict__ meta) {
+    const uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    const uint64_t chunk_lo = c * CNT_CHUNK;
+    const uint64_t chunk_hi = min(chunk_lo + CNT_CHUNK, n_bases);
+    int64_t c0 = last_le(base_off, (int64_t)n + 1, chunk_lo);
+    int64_t c1 = last_le(base_off, (int64_t)n + 1, chunk_hi - 1);
+    c0 = c0 < 0 ? 0 : c0;
+    c1 = c1 > (int64_t)n - 1 ? (int64_t)n - 1 : c1;
+    const uint64_t lo = chunk_lo >= 3 ? chunk_lo - 3 : 0;
+    const int64_t r0 = n_runs ? first_end_gt(runs, (int64_t)n_runs, lo) : 0;
+    const int64_t r1 = n_runs ? first_start_ge(runs, (int64_t)n_runs, chunk_hi) : 0;
+    meta[c] = make_longlong4(c0, c1, r0, r1);
+}
+
 __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __restrict__ packed, uint64_t n_bases,
-                                                              const uint64_t* __restrict__ runs, uint64_t n_runs,
-                                                              const uint64_t* __restrict__ base_off, uint32_t n,
+                                                              const uint64_t* __restrict__ runs,
+                                                              const uint64_t* __restrict__ base_off,
+                                                              const longlong4* __restrict__ meta,
                                                               uint32_t* __restrict__ counts) {
     __shared__ uint32_t hist[CNT_MAXSEG][256];
     __shared__ uint32_t fold[CNT_MAXSEG][136];
     __shared__ uint64_t seg_lo[CNT_MAXSEG + 1];
-    __shared__ int64_t s_c[2];
-    __shared__ int64_t s_r[2];
 
     const int tid = threadIdx.x;
     const uint64_t chunk_lo = (uint64_t)blockIdx.x * CNT_CHUNK;
     const uint64_t chunk_hi = min(chunk_lo + CNT_CHUNK, n_bases);
-
-    if (tid == 0) {
-        // contigs holding the first and last window end of the chunk
-        s_c[0] = last_le(base_off, (int64_t)n + 1, chunk_lo);
-        s_c[1] = last_le(base_off, (int64_t)n + 1, chunk_hi - 1);
-    } else if (tid == 64) {
-        uint64_t lo = chunk_lo >= 3 ? chunk_lo - 3 : 0;
-        s_r[0] = n_runs ? first_end_gt(runs, (int64_t)n_runs, lo) : 0;
-        s_r[1] = n_runs ? first_start_ge(runs, (int64_t)n_runs, chunk_hi) : 0;
-    }
-    __syncthreads();
-    int64_t c_first = s_c[0] < 0 ? 0 : s_c[0];
-    int64_t c_last = s_c[1] > (int64_t)n - 1 ? (int64_t)n - 1 : s_c[1];
-    const int64_t r_first = s_r[0], r_last = s_r[1];
+    const longlong4 mt = meta[blockIdx.x];
+    const int64_t c_first = mt.x, c_last = mt.y;
+    const int64_t r_first = mt.z, r_last = mt.w;
 
     const uint64_t w0 = chunk_lo / 16;
     for (int64_t cs = c_first; cs <= c_last; cs += CNT_MAXSEG) {
@@ -284,9 +293,14 @@ int mdet_tnf_count(mdet_ctx* ctx, const uint32_t* packed_dev, uint64_t n_bases, 
     MDET_HIP(hipMemsetAsync(counts, 0, (size_t)n * 136 * sizeof(uint32_t), ctx->stream));
     const uint64_t nchunks = (n_bases + CNT_CHUNK - 1) / CNT_CHUNK;
     if (nchunks > 0x7fffffffu) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_count: %llu bases is too many", (unsigned long long)n_bases);
-    if (nchunks)
+    if (nchunks) {
+        longlong4* meta;
+        MDET_TRY(ctx_buf(ctx, "count.meta", (size_t)nchunks, &meta));
+        hipLaunchKernelGGL(k_chunk_meta, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0, ctx->stream, n_bases,
+                           runs_dev, n_runs, base_off_dev, n, nchunks, meta);
         hipLaunchKernelGGL(k_count_packed, dim3((unsigned)nchunks), dim3(CNT_THREADS), 0, ctx->stream, packed_dev,
-                           n_bases, runs_dev, n_runs, base_off_dev, n, counts);
+                           n_bases, runs_dev, base_off_dev, meta, counts);
+    }
     MDET_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_normalize, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, counts, n, tnf_dev);
     MDET_HIP(hipGetLastError());
