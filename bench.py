@@ -237,7 +237,8 @@ def main():
     ap.add_argument("--modes", default="exact_f32,f16_refine")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--n-probe", type=int, default=16, help="clusters probed per cluster by the learned filter")
+    ap.add_argument("--n-probe", type=int, default=0,
+                    help="clusters probed per cluster by the learned filter (0: 16 up to 262,144 contigs, else 32)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain copy of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -292,7 +293,8 @@ def main():
         modes["bf16_refine"]["speedup_vs_exact_f32"] = modes["bf16_refine"]["pairs_per_s"] / modes["exact_f32"]["pairs_per_s"]
     if not args.no_learned:
         log(f"learned filter: {args.train_steps} training steps")
-        lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, args.n_probe)
+        n_probe = args.n_probe or (16 if w.n <= 262144 else 32)
+        lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, n_probe)
         lm["speedup_vs_bf16_refine"] = lm["pairs_per_s"] / modes["bf16_refine"]["pairs_per_s"]
         modes["learned"] = lm
 

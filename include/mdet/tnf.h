@@ -227,7 +227,7 @@ int mdet_mlp_embed(mdet_mlp *mlp, const float *tnf_dev, uint32_t n, float *emb_d
  * is below radius2 are evaluated exactly (the same refinement and selection
  * as the other modes).  Candidate pairs: with n_probe == 0 every pair (MFMA
  * Gram of the 32-dimensional embeddings over the whole triangle); with
- * n_probe > 0 (at most 16) a cluster index over the embeddings (k-means,
+ * n_probe > 0 (at most 32) a cluster index over the embeddings (k-means,
  * n_clusters centroids, 0 = n/192, at most 8192) and only the pairs of
  * clusters among each other's n_probe nearest centroids.  Edges of pairs the filter fails to
  * propose are missed: recall is measured against mdet_tnf_edges.

@@ -33,7 +33,7 @@ struct IvfIndex {
 };
 
 // Device part: clusters k (0: n / 192, at most 8192), probes 1 <= n_probe <=
-// 16; sets perm and queues the copies of the cluster offsets and probe lists
+// 32; sets perm and queues the copies of the cluster offsets and probe lists
 // to pinned host memory.  Work queued after it overlaps ivf_finish's host part.
 int ivf_launch(mdet_ctx* ctx, const float* emb_dev, uint32_t n, uint32_t k, uint32_t n_probe, IvfIndex* out);
 // Host part: waits for those copies only, builds the items and queues their

@@ -7,7 +7,7 @@
 // most partner pairs in the same cluster, and almost all of the rest in one of
 // the few clusters whose centroids are nearest.  So the learned path evaluates
 // its embedding Gram only on the pairs of probed cluster pairs:
-//   1. k-means (Lloyd) on a sample of the rows, 4 iterations; centroid sums
+//   1. k-means (Lloyd) on a sample of the rows, 8 iterations; centroid sums
 //      in 2^-20 fixed point (integer atomics: the result does not depend on
 //      the order of the additions), then one assignment of every row;
 //   2. rows sorted by cluster (radix sort, stable), cluster offsets;
@@ -29,11 +29,11 @@ namespace {
 constexpr int E_D = 32;                 // embedding dimensions
 constexpr uint32_t KM_MAX_K = 8192;
 constexpr double KM_FIX = 1048576.0;    // fixed-point scale of the centroid sums
-// Lloyd iterations on a sample of 16 rows per centroid: on trained 200k-contig
-// embeddings 4 x 16 keeps as many partner pairs within 8 probes as 8 x 32
-// (measured), at a quarter of the cost; 2 iterations lose recall.
-constexpr int KM_ITERS = 4;
-constexpr uint32_t KM_SAMPLE_PER_CLUSTER = 16;
+// Lloyd iterations on a sample of 32 rows per centroid.  Measured edge recall
+// with 32 probes at 1M contigs (5208 centroids): 4 x 16 -> 0.9983, 8 x 32 ->
+// 0.9992, 8 x 64 -> 0.9993 (at 200k contigs 4 x 16 already matched 8 x 32).
+constexpr int KM_ITERS = 8;
+constexpr uint32_t KM_SAMPLE_PER_CLUSTER = 32;
 
 // centroid c <- row c * n / k (rows are in input order)
 __global__ void k_km_init(const float* __restrict__ emb, uint32_t n, uint32_t k, float* __restrict__ cent) {
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_km_update(unsigned long long* __restric
 // best of the centroids b = lane, lane + 64, ... in registers (insertion into
 // a sorted list of (distance, index)), then the wave sorts the 64 x n_probe
 // candidates in LDS and the first n_probe are the answer.
-constexpr uint32_t KM_MAX_PROBE = 16;
+constexpr uint32_t KM_MAX_PROBE = 32;
 
 __device__ inline void probe_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
