@@ -7,7 +7,7 @@
 // most partner pairs in the same cluster, and almost all of the rest in one of
 // the few clusters whose centroids are nearest.  So the learned path evaluates
 // its embedding Gram only on the pairs of probed cluster pairs:
-//   1. k-means (Lloyd) on a sample of the rows, 8 iterations; centroid sums
+//   1. k-means (Lloyd) on a sample of the rows, 4-8 iterations; centroid sums
 //      in 2^-20 fixed point (integer atomics: the result does not depend on
 //      the order of the additions), then one assignment of every row;
 //   2. rows sorted by cluster (radix sort, stable), cluster offsets;
@@ -29,11 +29,14 @@ namespace {
 constexpr int E_D = 32;                 // embedding dimensions
 constexpr uint32_t KM_MAX_K = 8192;
 constexpr double KM_FIX = 1048576.0;    // fixed-point scale of the centroid sums
-// Lloyd iterations on a sample of 32 rows per centroid.  Measured edge recall
-// with 32 probes at 1M contigs (5208 centroids): 4 x 16 -> 0.9983, 8 x 32 ->
-// 0.9992, 8 x 64 -> 0.9993 (at 200k contigs 4 x 16 already matched 8 x 32).
-constexpr int KM_ITERS = 8;
-constexpr uint32_t KM_SAMPLE_PER_CLUSTER = 32;
+// Lloyd iterations x sampled rows per centroid, by the number of centroids.
+// Measured: at 1M contigs (5208 centroids, 32 probes) the edge recall is
+// 0.9983 with 4 x 16, 0.9992 with 8 x 32, 0.9993 with 8 x 64; at 200k
+// contigs (1042 centroids) 4 x 16 keeps as many partner pairs as 8 x 32, and
+// the sampled passes (bound by the centroid-sum atomics) cost 1.3 ms more.
+constexpr uint32_t KM_FEW_CLUSTERS = 2048;
+inline int km_iters(uint32_t k) { return k <= KM_FEW_CLUSTERS ? 4 : 8; }
+inline uint32_t km_sample_per_cluster(uint32_t k) { return k <= KM_FEW_CLUSTERS ? 16 : 32; }
 
 // centroid c <- row c * n / k (rows are in input order)
 __global__ void k_km_init(const float* __restrict__ emb, uint32_t n, uint32_t k, float* __restrict__ cent) {
@@ -294,9 +297,9 @@ int ivf_launch(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t
     hipLaunchKernelGGL(k_km_norms, dim3((k + 255) / 256), dim3(256), 0, s, cent, k, cnorm);
     MDET_HIP(hipMemsetAsync(sums, 0, (size_t)k * E_D * 8, s));
     MDET_HIP(hipMemsetAsync(counts, 0, ((size_t)k + 1) * 4, s));
-    const uint32_t stride = std::max<uint32_t>(1, n / std::max<uint32_t>(1, k * KM_SAMPLE_PER_CLUSTER));
+    const uint32_t stride = std::max<uint32_t>(1, n / std::max<uint32_t>(1, k * km_sample_per_cluster(k)));
     const uint32_t m = (n - 1) / stride + 1;
-    for (int it = 0; it < KM_ITERS; ++it) {
+    for (int it = 0; it < km_iters(k); ++it) {
         hipLaunchKernelGGL(k_km_assign<16>, dim3((m + 63) / 64), dim3(64 * 16), 0, s, emb, m, stride, cent, cnorm, k,
                            (uint32_t*)nullptr, counts, sums, 2);
         hipLaunchKernelGGL(k_km_update, dim3((k + 7) / 8), dim3(256), 0, s, sums, counts, k, cent, cnorm);
