@@ -16,6 +16,7 @@
 // a training step is ~10^9 flops, so these kernels are launch-bound, not
 // compute-bound; the all-pairs embedding Gram that uses the network at
 // inference runs on MFMA (tnf_pairs.hip).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -46,23 +47,27 @@ __device__ inline float gelu_grad(float x) {
 // C[M x N] (+)= op(A) * op(B); A is M x K (row-major, lda) or K x M if TA;
 // B is K x N (row-major, ldb) or N x K if TB.  Epilogue: + bias[n], store
 // pre-activation to Z (optional) and act(.) to C (GELU if gelu_out).
+// Split K: block z covers k in [z * kchunk, (z + 1) * kchunk) and writes to
+// C + z * zstride (partial products, summed by k_sum_partials).
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, int lda, const float* __restrict__ B,
                                               int ldb, float* __restrict__ C, int ldc, float* __restrict__ Z, int M,
                                               int N, int K, const float* __restrict__ bias, int gelu_out,
-                                              int accumulate) {
+                                              int accumulate, int kchunk, size_t zstride) {
     __shared__ float As[16][64 + 1], Bs[16][64 + 1];
     const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    C += blockIdx.z * zstride;
     float acc[4][4] = {};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kb; k0 < ke; k0 += 16) {
         for (int e = tid; e < 16 * 64; e += 256) {
             const int kk = e >> 6, mm = e & 63;
             const int m = m0 + mm, k = k0 + kk;
             float va = 0.f, vb = 0.f;
-            if (m < M && k < K) va = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
+            if (m < M && k < ke) va = TA ? A[(size_t)k * lda + m] : A[(size_t)m * lda + k];
             const int n = n0 + mm;
-            if (n < N && k < K) vb = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
+            if (n < N && k < ke) vb = TB ? B[(size_t)n * ldb + k] : B[(size_t)k * ldb + n];
             As[kk][mm] = va;
             Bs[kk][mm] = vb;
         }
@@ -145,13 +150,34 @@ __global__ void k_gelu_back(const float* __restrict__ dA, const float* __restric
     if (k < count) dZ[k] = dA[k] * gelu_grad(Z[k]);
 }
 
-// db[n] = sum_m dZ[m][n]
-__global__ void k_colsum(const float* __restrict__ dZ, int M, int N, float* __restrict__ db) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// db[n] = sum_m dZ[m][n]: 64 columns x 16 row slices per block, the slices
+// summed in a fixed order.
+constexpr int CS_SLICES = 16;
+__global__ __launch_bounds__(64 * CS_SLICES) void k_colsum(const float* __restrict__ dZ, int M, int N,
+                                                          float* __restrict__ db) {
+    __shared__ float part[CS_SLICES][64];
+    const int c = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += dZ[(size_t)m * N + n];
-    db[n] = s;
+    if (n < N)
+        for (int m = sl; m < M; m += CS_SLICES) s += dZ[(size_t)m * N + n];
+    part[sl][c] = s;
+    __syncthreads();
+    if (sl == 0 && n < N) {
+        float t = 0.f;
+#pragma unroll
+        for (int q = 0; q < CS_SLICES; ++q) t += part[q][c];
+        db[n] = t;
+    }
+}
+
+// out[i] = sum over z < S of part[z * count + i], in order
+__global__ void k_sum_partials(const float* __restrict__ part, int S, size_t count, float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float t = 0.f;
+    for (int z = 0; z < S; ++z) t += part[(size_t)z * count + i];
+    out[i] = t;
 }
 
 // Adam on the FP32 master copy; writes the BF16-rounded weights the forward
@@ -398,18 +424,24 @@ int mlp_forward(mdet_mlp* m, const float* X, int M, float* Z1, float* A1, float*
     hipStream_t s = m->ctx->stream;
     const float* w = m->wq;
     hipLaunchKernelGGL((k_gemm<false, true>), gemm_grid(M, D_H), dim3(256), 0, s, X, D_IN, w + OFF_W1, D_IN, A1, D_H,
-                       Z1, M, D_H, D_IN, w + OFF_B1, 1, 0);
+                       Z1, M, D_H, D_IN, w + OFF_B1, 1, 0, D_IN, (size_t)0);
     hipLaunchKernelGGL((k_gemm<false, true>), gemm_grid(M, D_H), dim3(256), 0, s, A1, D_H, w + OFF_W2, D_H, A2, D_H, Z2,
-                       M, D_H, D_H, w + OFF_B2, 1, 0);
+                       M, D_H, D_H, w + OFF_B2, 1, 0, D_H, (size_t)0);
     hipLaunchKernelGGL((k_gemm<false, true>), gemm_grid(M, D_E), dim3(256), 0, s, A2, D_H, w + OFF_W3, D_H, E, D_E,
-                       (float*)nullptr, M, D_E, D_H, w + OFF_B3, 0, 0);
+                       (float*)nullptr, M, D_E, D_H, w + OFF_B3, 0, 0, D_H, (size_t)0);
     MDET_HIP(hipGetLastError());
     return MDET_OK;
 }
 
 struct Scratch {
-    float *X, *Z1, *A1, *Z2, *A2, *E, *dE, *dA2, *dZ2, *dA1, *dZ1, *loss;
+    float *X, *Z1, *A1, *Z2, *A2, *E, *dE, *dA2, *dZ2, *dA1, *dZ1, *loss, *part;
 };
+
+// Weight gradients sum over the batch's rows: split into at most WG_SPLITS
+// slices of at least WG_MIN_ROWS rows, one partial product each, summed in
+// slice order (deterministic for a given batch size).
+constexpr int WG_SPLITS = 32, WG_MIN_ROWS = 64;
+int wg_splits(int M) { return std::max(1, std::min(WG_SPLITS, M / WG_MIN_ROWS)); }
 
 int scratch(mdet_mlp* m, int M, Scratch* S) {
     mdet_ctx* c = m->ctx;
@@ -425,6 +457,7 @@ int scratch(mdet_mlp* m, int M, Scratch* S) {
     MDET_TRY(ctx_buf(c, "mlp.dA1", (size_t)M * D_H, &S->dA1));
     MDET_TRY(ctx_buf(c, "mlp.dZ1", (size_t)M * D_H, &S->dZ1));
     MDET_TRY(ctx_buf(c, "mlp.loss", 4, &S->loss));
+    MDET_TRY(ctx_buf(c, "mlp.part", (size_t)wg_splits(M) * std::max(D_H * D_H, D_H * D_IN), &S->part));
     return MDET_OK;
 }
 
@@ -440,25 +473,35 @@ int mlp_grad(mdet_mlp* m, const float* tnf, const uint32_t* pi, const uint32_t* 
     hipLaunchKernelGGL(k_pair_loss, dim3((B * 32 + 255) / 256), dim3(256), 0, s, S->E, y, B, margin, S->dE, S->loss);
     const float* w = m->wq;
     float* g = m->g;
+    // dW[rows x cols] = dZ^T A over the batch's M rows, split by rows
+    const int nsplit = wg_splits(M);
+    const int kchunk = ((M + nsplit - 1) / nsplit + 15) / 16 * 16;
+    auto weight_grad = [&](const float* dZ, int rows, const float* A, int cols, float* dW) {
+        dim3 grid = gemm_grid(rows, cols);
+        grid.z = (unsigned)nsplit;
+        const size_t count = (size_t)rows * cols;
+        hipLaunchKernelGGL((k_gemm<true, false>), grid, dim3(256), 0, s, dZ, rows, A, cols, S->part, cols,
+                           (float*)nullptr, rows, cols, M, (const float*)nullptr, 0, 0, kchunk, count);
+        hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, S->part, nsplit,
+                           count, dW);
+    };
+    const dim3 cs_block(64 * CS_SLICES);
     // layer 3: dW3 = dE^T A2, db3 = sum dE, dA2 = dE W3
-    hipLaunchKernelGGL((k_gemm<true, false>), gemm_grid(D_E, D_H), dim3(256), 0, s, S->dE, D_E, S->A2, D_H,
-                       g + OFF_W3, D_H, (float*)nullptr, D_E, D_H, M, (const float*)nullptr, 0, 0);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, s, S->dE, M, D_E, g + OFF_B3);
+    weight_grad(S->dE, D_E, S->A2, D_H, g + OFF_W3);
+    hipLaunchKernelGGL(k_colsum, dim3((D_E + 63) / 64), cs_block, 0, s, S->dE, M, D_E, g + OFF_B3);
     hipLaunchKernelGGL((k_gemm<false, false>), gemm_grid(M, D_H), dim3(256), 0, s, S->dE, D_E, w + OFF_W3, D_H, S->dA2,
-                       D_H, (float*)nullptr, M, D_H, D_E, (const float*)nullptr, 0, 0);
+                       D_H, (float*)nullptr, M, D_H, D_E, (const float*)nullptr, 0, 0, D_E, (size_t)0);
     const uint64_t cnt = (uint64_t)M * D_H;
     hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, S->dA2, S->Z2, S->dZ2, cnt);
     // layer 2
-    hipLaunchKernelGGL((k_gemm<true, false>), gemm_grid(D_H, D_H), dim3(256), 0, s, S->dZ2, D_H, S->A1, D_H,
-                       g + OFF_W2, D_H, (float*)nullptr, D_H, D_H, M, (const float*)nullptr, 0, 0);
-    hipLaunchKernelGGL(k_colsum, dim3(4), dim3(64), 0, s, S->dZ2, M, D_H, g + OFF_B2);
+    weight_grad(S->dZ2, D_H, S->A1, D_H, g + OFF_W2);
+    hipLaunchKernelGGL(k_colsum, dim3((D_H + 63) / 64), cs_block, 0, s, S->dZ2, M, D_H, g + OFF_B2);
     hipLaunchKernelGGL((k_gemm<false, false>), gemm_grid(M, D_H), dim3(256), 0, s, S->dZ2, D_H, w + OFF_W2, D_H,
-                       S->dA1, D_H, (float*)nullptr, M, D_H, D_H, (const float*)nullptr, 0, 0);
+                       S->dA1, D_H, (float*)nullptr, M, D_H, D_H, (const float*)nullptr, 0, 0, D_H, (size_t)0);
     hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, S->dA1, S->Z1, S->dZ1, cnt);
     // layer 1
-    hipLaunchKernelGGL((k_gemm<true, false>), gemm_grid(D_H, D_IN), dim3(256), 0, s, S->dZ1, D_H, S->X, D_IN,
-                       g + OFF_W1, D_IN, (float*)nullptr, D_H, D_IN, M, (const float*)nullptr, 0, 0);
-    hipLaunchKernelGGL(k_colsum, dim3(4), dim3(64), 0, s, S->dZ1, M, D_H, g + OFF_B1);
+    weight_grad(S->dZ1, D_H, S->X, D_IN, g + OFF_W1);
+    hipLaunchKernelGGL(k_colsum, dim3((D_H + 63) / 64), cs_block, 0, s, S->dZ1, M, D_H, g + OFF_B1);
     MDET_HIP(hipGetLastError());
     return MDET_OK;
 }
