@@ -7,9 +7,10 @@
 // most partner pairs in the same cluster, and almost all of the rest in one of
 // the few clusters whose centroids are nearest.  So the learned path evaluates
 // its embedding Gram only on the pairs of probed cluster pairs:
-//   1. k-means (Lloyd) on a sample of the rows, 4-8 iterations; centroid sums
-//      in 2^-20 fixed point (integer atomics: the result does not depend on
-//      the order of the additions), then one assignment of every row;
+//   1. k-means (Lloyd) on a sample of the rows, 4-8 iterations, distances on
+//      FP16 MFMA; centroid sums in 2^-20 fixed point (integer atomics: the
+//      result does not depend on the order of the additions), then one
+//      assignment of every row;
 //   2. rows sorted by cluster (radix sort, stable), cluster offsets;
 //   3. per cluster the n_probe nearest non-empty centroids (itself first);
 //   4. on the host: the distinct unordered cluster pairs (a, b) with b in a's
@@ -46,136 +47,312 @@ __global__ void k_km_init(const float* __restrict__ emb, uint32_t n, uint32_t k,
     cent[(size_t)c * E_D + d] = emb[src * E_D + d];
 }
 
-__global__ void k_km_norms(const float* __restrict__ cent, uint32_t k, float* __restrict__ cnorm) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= k) return;
-    float s = 0.f;
-#pragma unroll
-    for (int d = 0; d < E_D; ++d) s = fmaf(cent[(size_t)c * E_D + d], cent[(size_t)c * E_D + d], s);
-    cnorm[c] = s;
+// ---- assignment on MFMA (v_mfma_f32_32x32x16_f16, FP32 accumulate)
+// Coordinates are split into FP16 pairs hi + lo (hi = fp16(v), lo =
+// fp16(v - hi)), and the MFMA accumulates, for a point x and a centroid c,
+//   -xh.ch - xh.cl - xl.ch                       (the dropped xl.cl and the
+//                                                 FP32 sums: ~2^-21 relative)
+//   + 1 * (hi, lo halves of s^2 |c - o|^2 / 2)
+//   + (s^2 |x - o|^2 / 2 * (1 + 2^-6) + 2^-8) * 1
+// over x, c in the frame (origin o = mean of the initial centroids, scale
+// s = 2^e with s |x - o| <= KM_SPAN for every point, so all operands are in
+// the FP16 range; centroids stay in the points' hull).  That is
+// s^2 |x - c|^2 / 2 + delta_x, with delta_x > 0 far above the rounding error
+// wherever |x - c| is small, so every value is positive and its float bits
+// order like the values.  A point's nearest centroid is the smallest key
+// (value bits with the low 8 bits replaced by the 32-centroid tile index),
+// then the smallest index among equal keys: values within 2^-15 relative of
+// each other count as equal.
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// fragments per 32 points or centroids: hi dims 0-15, 16-31, lo dims 0-15,
+// 16-31, the norm slots
+constexpr int KM_KF = 5;
+constexpr float KM_SPAN = 128.f;  // bound on s |x - o|
+constexpr uint32_t KM_TILES_MAX = KM_MAX_K / 32;
+static_assert(KM_TILES_MAX <= 256, "tile index must fit the key's low 8 bits");
+
+struct KmFrame {
+    float o[E_D];
+    unsigned int maxd2;  // float bits of max |x - o|^2 over the points
+};
+
+__device__ inline float km_scale(const KmFrame* fr) {
+    const float m = __uint_as_float(fr->maxd2);
+    if (!(m > 0.f) || !(m < __builtin_inff())) return 1.f;
+    const int e2 = ilogbf(KM_SPAN * KM_SPAN / m);  // 2^e2 <= SPAN^2 / m
+    const int e = e2 >= 0 ? e2 / 2 : -((1 - e2) / 2);  // floor(e2 / 2)
+    return ldexpf(1.f, e);
 }
 
-// Nearest centroid (smallest |c|^2 - 2 x.c, lowest index on ties) of the
-// points q * stride, q < m.  Block = 64 points x KM_WAVES waves: lane l of
-// every wave holds point l's 32 coordinates in registers, wave w scans the
-// centroids c = w, w + KM_WAVES, ... (their coordinates are wave-uniform:
-// scalar loads, FMAs from SGPRs; two centroids per iteration on independent
-// accumulator chains), and the partial minima are merged in LDS.  More waves
-// per point for the small sampled passes, whose latency would otherwise be
-// exposed.  mode 1: counts[best] += 1;
-// mode 2: also the point's coordinates into the fixed-point sums of its
-// centroid.
-template <int KM_WAVES>
-__global__ __launch_bounds__(64 * KM_WAVES) void k_km_assign(const float* __restrict__ emb, uint32_t m,
-                                                            uint32_t stride, const float* __restrict__ cent,
-                                                            const float* __restrict__ cnorm, uint32_t k,
-                                                            uint32_t* __restrict__ assign,
-                                                            uint32_t* __restrict__ counts,
-                                                            unsigned long long* __restrict__ sums, int mode) {
-    __shared__ float sb[KM_WAVES][64];
-    __shared__ uint32_t si[KM_WAVES][64];
-    // wave-uniform by construction; readfirstlane makes it an SGPR value, so
-    // the centroid addresses are uniform and load through the scalar unit
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const uint32_t q = blockIdx.x * 64 + lane;
-    const bool valid = q < m;
-    const size_t p = (size_t)(valid ? q : 0) * stride;
-    float x[E_D];
+__device__ inline f32x16_t km_mfma(uint4 a, uint4 b, f32x16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b), c,
+                                                  0, 0, 0);
+}
+
+__device__ inline uint4 km_pack(const float* v) {
+    uint32_t w[4];
 #pragma unroll
-    for (int d = 0; d < E_D; ++d) x[d] = emb[p * E_D + d];
-    float best = __builtin_inff();
-    uint32_t bi = 0xffffffffu;
-    uint32_t c = wave;
-    for (; c + KM_WAVES < k; c += 2 * KM_WAVES) {
-        const float* c0 = cent + (size_t)c * E_D;
-        const float* c1 = c0 + KM_WAVES * E_D;
-        float u0 = 0.f, u1 = 0.f, v0 = 0.f, v1 = 0.f;
-#pragma unroll
-        for (int d = 0; d < E_D; d += 2) {
-            u0 = fmaf(x[d], c0[d], u0);
-            u1 = fmaf(x[d + 1], c0[d + 1], u1);
-            v0 = fmaf(x[d], c1[d], v0);
-            v1 = fmaf(x[d + 1], c1[d + 1], v1);
-        }
-        const float du = fmaf(-2.f, u0 + u1, cnorm[c]);
-        const float dv = fmaf(-2.f, v0 + v1, cnorm[c + KM_WAVES]);
-        if (du < best) {
-            best = du;
-            bi = c;
-        }
-        if (dv < best) {
-            best = dv;
-            bi = c + KM_WAVES;
-        }
+    for (int e = 0; e < 4; ++e) {
+        const _Float16 lo = (_Float16)v[2 * e], hi = (_Float16)v[2 * e + 1];
+        w[e] = (uint32_t)__builtin_bit_cast(uint16_t, lo) | ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
     }
-    if (c < k) {
-        const float* c0 = cent + (size_t)c * E_D;
-        float u0 = 0.f, u1 = 0.f;
-#pragma unroll
-        for (int d = 0; d < E_D; d += 2) {
-            u0 = fmaf(x[d], c0[d], u0);
-            u1 = fmaf(x[d + 1], c0[d + 1], u1);
-        }
-        const float du = fmaf(-2.f, u0 + u1, cnorm[c]);
-        if (du < best) {
-            best = du;
-            bi = c;
-        }
-    }
-    sb[wave][lane] = best;
-    si[wave][lane] = bi;
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// o = mean of the k centroids (fixed summation order); clears maxd2.  One block of 1024.
+__global__ __launch_bounds__(1024) void k_km_center(const float* __restrict__ cent, uint32_t k, KmFrame* __restrict__ fr) {
+    __shared__ float part[32][E_D];
+    const int d = threadIdx.x % E_D, ch = threadIdx.x / E_D;
+    float sum = 0.f;
+#pragma unroll 8
+    for (uint32_t c = ch; c < k; c += 32) sum += cent[(size_t)c * E_D + d];
+    part[ch][d] = sum;
     __syncthreads();
-    if (wave != 0 || !valid) return;
+    if (threadIdx.x < E_D) {
+        float t = 0.f;
+        for (int q = 0; q < 32; ++q) t += part[q][threadIdx.x];
+        fr->o[threadIdx.x] = t / (float)k;
+        if (threadIdx.x == 0) fr->maxd2 = 0u;
+    }
+}
+
+// maxd2 = max over the n points of |x - o|^2 (order-free maximum of
+// non-negative float bits; grid-stride, one atomic per block)
+__global__ __launch_bounds__(256) void k_km_extent(const float* __restrict__ emb, uint32_t n, KmFrame* __restrict__ fr) {
+    __shared__ uint32_t wmax[4];
+    uint32_t u = 0;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const float4* x4 = reinterpret_cast<const float4*>(emb + (size_t)q * E_D);
+        float d2 = 0.f;
 #pragma unroll
-    for (int w = 1; w < KM_WAVES; ++w) {
-        const float b2 = sb[w][lane];
-        const uint32_t i2 = si[w][lane];
-        if (b2 < best || (b2 == best && i2 < bi)) {
-            best = b2;
-            bi = i2;
+        for (int v = 0; v < E_D / 4; ++v) {
+            const float4 x = x4[v];
+            const float a = x.x - fr->o[4 * v], b = x.y - fr->o[4 * v + 1], c = x.z - fr->o[4 * v + 2],
+                        e = x.w - fr->o[4 * v + 3];
+            d2 = fmaf(a, a, fmaf(b, b, fmaf(c, c, fmaf(e, e, d2))));
+        }
+        u = max(u, __float_as_uint(d2));  // NaN rows: bits above +inf, scale falls back to 1
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) u = max(u, (uint32_t)__shfl_xor((int)u, off, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t b = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        if (b) atomicMax(&fr->maxd2, b);
+    }
+}
+
+__device__ inline void km_split(float v, float& hi, float& lo) {
+    hi = (float)(_Float16)v;
+    lo = v - hi;
+}
+
+// Centroid fragments: tile t (centroids 32t..32t+31) x KM_KF x 64 lanes;
+// lane l holds column l % 32, k = 8 (l / 32) + 0..7 of the fragment's 16.
+// Coordinates enter negated.  Centroids past k get an unbeatable norm slot.
+__global__ __launch_bounds__(256) void k_km_cfrag(const float* __restrict__ cent, uint32_t k,
+                                                  const KmFrame* __restrict__ fr, uint4* __restrict__ cfrag) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ntiles = (k + 31) / 32;
+    if (g >= ntiles * 64) return;
+    const uint32_t t = g / 64, lane = g % 64;
+    const uint32_t c = t * 32 + (lane & 31), half = lane >> 5;
+    const float s = km_scale(fr);
+    float vh[8], vl[8], v[8];
+    uint4 f[KM_KF];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = 0.f;
+    if (c < k) {
+        const float* cr = cent + (size_t)c * E_D;
+        float n2 = 0.f;
+        for (int d = 0; d < E_D; ++d) {
+            const float y = s * (cr[d] - fr->o[d]);
+            n2 = fmaf(y, y, n2);
+        }
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) km_split(-s * (cr[16 * st + 8 * half + e] - fr->o[16 * st + 8 * half + e]),
+                                                 vh[e], vl[e]);
+            f[st] = km_pack(vh);
+            f[2 + st] = km_pack(vl);
+        }
+        if (half == 0) {
+            km_split(0.5f * n2, v[0], v[1]);
+            v[2] = 1.f;
+        }
+    } else {
+        f[0] = f[1] = f[2] = f[3] = make_uint4(0, 0, 0, 0);
+        if (half == 0) {
+            v[0] = 65504.f;
+            v[1] = 65504.f;
+            v[2] = 1.f;
         }
     }
-    if (bi >= k) bi = 0;  // k >= 1; only reachable with non-finite coordinates
-    if (assign) assign[q] = bi;
-    if (mode >= 1) atomicAdd(&counts[bi], 1u);
-    if (mode >= 2)
+    f[4] = km_pack(v);
 #pragma unroll
-        for (int d = 0; d < E_D; ++d)
-            atomicAdd(&sums[(size_t)bi * E_D + d], (unsigned long long)llrint((double)x[d] * KM_FIX));
+    for (int q = 0; q < KM_KF; ++q) cfrag[((size_t)t * KM_KF + q) * 64 + lane] = f[q];
+}
+
+// Nearest centroid of the points q * stride, q < m.  Block = 32 R points x W
+// waves: every wave holds the points' A fragments and scans the centroid
+// tiles t = w, w + W, ...; per element one AND-OR (key) and a running
+// minimum; the 32 lanes of a column half then reduce each row, and the W
+// waves' results are merged in LDS.  mode 1: counts[best] += 1; mode 2: also
+// the point's coordinates into the fixed-point sums of its centroid.  err
+// bit 0: a point outside the frame (only non-finite or overflowing rows).
+template <int R, int W>
+__global__ __launch_bounds__(64 * W) void k_km_assign(const float* __restrict__ emb, uint32_t m, uint32_t stride,
+                                                      const KmFrame* __restrict__ fr, const uint4* __restrict__ cfrag,
+                                                      uint32_t k, uint32_t* __restrict__ assign,
+                                                      uint32_t* __restrict__ counts,
+                                                      unsigned long long* __restrict__ sums, int mode,
+                                                      unsigned int* __restrict__ err) {
+    __shared__ uint32_t sv[W][32 * R], si[W][32 * R];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint32_t half = lane >> 5, p0 = blockIdx.x * 32 * R;
+    const float s = km_scale(fr);
+    // a[r]: hi dims 0-15, hi 16-31, lo 0-15, lo 16-31, norm slots
+    uint4 a[R][KM_KF];
+    bool bad = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t q = p0 + 32 * r + (lane & 31);
+        const bool valid = q < m;
+        const float* x = emb + (size_t)(valid ? q : 0) * stride * E_D;
+        float vh[8], vl[8], n2 = 0.f;
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int d = 16 * st + 8 * half + e;
+                const float v = valid ? s * (x[d] - fr->o[d]) : 0.f;
+                n2 = fmaf(v, v, n2);
+                km_split(v, vh[e], vl[e]);
+            }
+            a[r][st] = km_pack(vh);
+            a[r][2 + st] = km_pack(vl);
+        }
+        n2 += __shfl_xor(n2, 32, 64);
+        bad |= valid && !(n2 <= KM_SPAN * KM_SPAN * 1.0001f);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) vh[e] = 0.f;
+        if (half == 0) {
+            vh[0] = 1.f;
+            vh[1] = 1.f;
+            vh[2] = fmaf(0.5f * n2, 1.f + 0x1p-6f, 0x1p-8f);
+        }
+        a[r][4] = km_pack(vh);
+    }
+    if (bad) atomicOr(err, 1u);
+    uint32_t best[R][16];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) best[r][e] = 0xffffffffu;
+    const uint32_t ntiles = (k + 31) / 32;
+    for (uint32_t t = wave; t < ntiles; t += W) {
+        const uint4* bt = cfrag + (size_t)t * KM_KF * 64 + lane;
+        uint4 b[KM_KF];
+#pragma unroll
+        for (int f = 0; f < KM_KF; ++f) b[f] = bt[64 * f];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            f32x16_t acc = {};
+            acc = km_mfma(a[r][0], b[0], acc);  // xh.ch
+            acc = km_mfma(a[r][1], b[1], acc);
+            acc = km_mfma(a[r][0], b[2], acc);  // xh.cl
+            acc = km_mfma(a[r][1], b[3], acc);
+            acc = km_mfma(a[r][2], b[0], acc);  // xl.ch
+            acc = km_mfma(a[r][3], b[1], acc);
+            acc = km_mfma(a[r][4], b[4], acc);  // norm slots
+#pragma unroll
+            for (int e = 0; e < 16; ++e) best[r][e] = min(best[r][e], (__float_as_uint(acc[e]) & 0xffffff00u) | t);
+        }
+    }
+    // per row: (value bits, centroid) minimum over the 32 lanes of the half
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t key = best[r][e];
+            uint32_t v = key & 0xffffff00u;
+            uint32_t c = key == 0xffffffffu ? 0xffffffffu : (key & 0xffu) * 32 + (lane & 31);
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                const uint32_t v2 = (uint32_t)__shfl_xor((int)v, off, 64);
+                const uint32_t c2 = (uint32_t)__shfl_xor((int)c, off, 64);
+                if (v2 < v || (v2 == v && c2 < c)) {
+                    v = v2;
+                    c = c2;
+                }
+            }
+            if ((lane & 31) == e) {
+                const uint32_t row = 32 * r + 8 * (e / 4) + 4 * half + (e % 4);
+                sv[wave][row] = v;
+                si[wave][row] = c;
+            }
+        }
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t < 32 * R) {
+        uint32_t bv = sv[0][t], bi = si[0][t];
+#pragma unroll
+        for (int w = 1; w < W; ++w) {
+            const uint32_t v2 = sv[w][t], c2 = si[w][t];
+            if (v2 < bv || (v2 == bv && c2 < bi)) {
+                bv = v2;
+                bi = c2;
+            }
+        }
+        if (bi >= k) bi = 0;  // only reachable with non-finite coordinates
+        const uint32_t q = p0 + t;
+        if (q < m) {
+            if (assign) assign[q] = bi;
+            if (mode >= 1) atomicAdd(&counts[bi], 1u);
+        }
+        si[0][t] = bi;
+    }
+    if (mode < 2) return;
+    __syncthreads();
+    // fixed-point sums: lanes of an instruction cover 2 points x 32
+    // consecutive dimensions (contiguous centroid rows)
+    for (uint32_t g = threadIdx.x; g < 32 * R * E_D; g += 64 * W) {
+        const uint32_t pt = g / E_D, d = g % E_D, q = p0 + pt;
+        if (q < m)
+            atomicAdd(&sums[(size_t)si[0][pt] * E_D + d],
+                      (unsigned long long)llrint((double)emb[(size_t)q * stride * E_D + d] * KM_FIX));
+    }
 }
 
 // centroid <- mean of its points (unchanged if it has none); clears the sums
 // and counts for the next iteration.  32 threads (one per dimension) per
-// centroid, |c|^2 by a half-wave reduction.
+// centroid.
 __global__ __launch_bounds__(256) void k_km_update(unsigned long long* __restrict__ sums, uint32_t* __restrict__ counts,
-                                                   uint32_t k, float* __restrict__ cent, float* __restrict__ cnorm) {
+                                                   uint32_t k, float* __restrict__ cent) {
     const uint32_t c = blockIdx.x * (blockDim.x / E_D) + threadIdx.x / E_D;
     const uint32_t d = threadIdx.x % E_D;
     const bool ok = c < k;
     const uint32_t cnt = ok ? counts[c] : 0u;
-    float cv = 0.f;
     if (ok) {
         const size_t e = (size_t)c * E_D + d;
         const long long v = (long long)sums[e];
         sums[e] = 0ull;
-        cv = cnt ? (float)((double)v / KM_FIX / (double)cnt) : cent[e];
-        cent[e] = cv;
+        if (cnt) cent[e] = (float)((double)v / KM_FIX / (double)cnt);
     }
-    float s2 = cv * cv;
-#pragma unroll
-    for (int off = E_D / 2; off >= 1; off >>= 1) s2 += __shfl_xor(s2, off, E_D);
     __syncthreads();  // every lane has read counts[c] before it is cleared
-    if (ok && d == 0) {
-        cnorm[c] = s2;
-        counts[c] = 0;
-    }
+    if (ok && d == 0) counts[c] = 0;
 }
 
 // Per cluster a (one wave; 4 clusters per block): its n_probe (<= KM_MAX_PROBE)
-// nearest non-empty centroids, itself first.  Each lane keeps the n_probe
-// best of the centroids b = lane, lane + 64, ... in registers (insertion into
-// a sorted list of (distance, index)), then the wave sorts the 64 x n_probe
-// candidates in LDS and the first n_probe are the answer.
+// nearest non-empty centroids, itself first, by (distance, index).  Each lane
+// keeps the KM_MAX_PROBE best of the centroids b = lane, lane + 64, ... in
+// registers (insertion into a sorted list; b increases, so ties keep the
+// lower index), the lists go to LDS, and n_probe rounds take the wave-wide
+// minimum of the lanes' list heads, the owning lane advancing its head.
 constexpr uint32_t KM_MAX_PROBE = 32;
 
 __device__ inline void probe_wave_sync() {
@@ -186,11 +363,14 @@ __device__ inline void probe_wave_sync() {
 
 __global__ __launch_bounds__(256) void k_km_probe(const float* __restrict__ cent, const uint32_t* __restrict__ counts,
                                                   uint32_t k, uint32_t n_probe, uint32_t* __restrict__ probes) {
-    __shared__ float dk[4][64 * KM_MAX_PROBE];
-    __shared__ uint32_t di[4][64 * KM_MAX_PROBE];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ float dk[4][KM_MAX_PROBE * 64];
+    __shared__ uint32_t di[4][KM_MAX_PROBE * 64];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const uint32_t a = blockIdx.x * 4 + wave;
     if (a >= k) return;  // no block-wide synchronisation below
+    float ca[E_D];
+#pragma unroll
+    for (int d = 0; d < E_D; ++d) ca[d] = cent[(size_t)a * E_D + d];
     float bd[KM_MAX_PROBE];
     uint32_t bx[KM_MAX_PROBE];
 #pragma unroll
@@ -205,12 +385,11 @@ __global__ __launch_bounds__(256) void k_km_probe(const float* __restrict__ cent
             float s = 0.f;
 #pragma unroll
             for (int d = 0; d < E_D; ++d) {
-                const float t = cent[(size_t)a * E_D + d] - cent[(size_t)b * E_D + d];
+                const float t = ca[d] - cent[(size_t)b * E_D + d];
                 s = fmaf(t, t, s);
             }
             dist = s;
         }
-        // insert (dist, b) into the sorted list (b increases: ties keep the earlier)
         if (dist < bd[KM_MAX_PROBE - 1]) {
             float cd = dist;
             uint32_t cx = b;
@@ -229,34 +408,31 @@ __global__ __launch_bounds__(256) void k_km_probe(const float* __restrict__ cent
     }
     float* sd = dk[wave];
     uint32_t* sx = di[wave];
-    const uint32_t sz = 64 * KM_MAX_PROBE;
 #pragma unroll
     for (uint32_t t = 0; t < KM_MAX_PROBE; ++t) {
-        sd[lane * KM_MAX_PROBE + t] = bd[t];
-        sx[lane * KM_MAX_PROBE + t] = bx[t];
+        sd[t * 64 + lane] = bd[t];
+        sx[t * 64 + lane] = bx[t];
     }
     probe_wave_sync();
-    for (uint32_t s2 = 2; s2 <= sz; s2 <<= 1)
-        for (uint32_t j = s2 >> 1; j > 0; j >>= 1) {
-            for (uint32_t t = lane; t < sz; t += 64) {
-                const uint32_t u = t ^ j;
-                if (u > t) {
-                    const bool up = (t & s2) == 0;
-                    const bool before = sd[u] < sd[t] || (sd[u] == sd[t] && sx[u] < sx[t]);
-                    if (before == up) {
-                        const float td = sd[t];
-                        sd[t] = sd[u];
-                        sd[u] = td;
-                        const uint32_t tx = sx[t];
-                        sx[t] = sx[u];
-                        sx[u] = tx;
-                    }
-                }
+    // every centroid index sits in exactly one lane's list, so one lane holds the minimum
+    uint32_t head = 0;
+    for (uint32_t r = 0; r < n_probe; ++r) {
+        const float v = head < KM_MAX_PROBE ? sd[head * 64 + lane] : __builtin_inff();
+        const uint32_t x = head < KM_MAX_PROBE ? sx[head * 64 + lane] : 0xffffffffu;
+        float mv = v;
+        uint32_t mx = x;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const float v2 = __shfl_xor(mv, off, 64);
+            const uint32_t x2 = (uint32_t)__shfl_xor((int)mx, off, 64);
+            if (v2 < mv || (v2 == mv && x2 < mx)) {
+                mv = v2;
+                mx = x2;
             }
-            probe_wave_sync();
         }
-    for (uint32_t t = lane; t < n_probe; t += 64)
-        probes[(size_t)a * n_probe + t] = sd[t] < __builtin_inff() ? sx[t] : 0xffffffffu;
+        if (v == mv && x == mx) ++head;
+        if (lane == 0) probes[(size_t)a * n_probe + r] = mv < __builtin_inff() ? mx : 0xffffffffu;
+    }
 }
 
 __global__ void k_iota(uint32_t* __restrict__ v, uint32_t n) {
@@ -278,11 +454,16 @@ int ivf_launch(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t
         MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges_learned: n_probe %u above %u", n_probe, KM_MAX_PROBE);
     n_probe = std::min(n_probe, k);
     hipStream_t s = ctx->stream;
-    float *cent, *cnorm;
-    uint32_t *counts, *assign, *keys_sorted, *idx, *perm, *off, *probes;
+    float* cent;
+    uint32_t *counts, *assign, *keys_sorted, *idx, *perm, *off, *probes, *err;
     unsigned long long* sums;
+    KmFrame* frame;
+    uint4* cfrag;
+    const uint32_t ntiles = (k + 31) / 32;
     MDET_TRY(ctx_buf(ctx, "ivf.cent", (size_t)k * E_D, &cent));
-    MDET_TRY(ctx_buf(ctx, "ivf.cnorm", (size_t)k, &cnorm));
+    MDET_TRY(ctx_buf(ctx, "ivf.frame", 1, &frame));
+    MDET_TRY(ctx_buf(ctx, "ivf.cfrag", (size_t)ntiles * KM_KF * 64, &cfrag));
+    MDET_TRY(ctx_buf(ctx, "ivf.err", 1, &err));
     MDET_TRY(ctx_buf(ctx, "ivf.counts", (size_t)k + 1, &counts));
     MDET_TRY(ctx_buf(ctx, "ivf.sums", (size_t)k * E_D, &sums));
     MDET_TRY(ctx_buf(ctx, "ivf.assign", (size_t)n, &assign));
@@ -294,18 +475,32 @@ int ivf_launch(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t
 
     // 1. k-means on a sample (every stride-th row), then every row
     hipLaunchKernelGGL(k_km_init, dim3(k), dim3(E_D), 0, s, emb, n, k, cent);
-    hipLaunchKernelGGL(k_km_norms, dim3((k + 255) / 256), dim3(256), 0, s, cent, k, cnorm);
+    hipLaunchKernelGGL(k_km_center, dim3(1), dim3(1024), 0, s, cent, k, frame);
+    hipLaunchKernelGGL(k_km_extent, dim3(std::min<uint32_t>((n + 255) / 256, 4u * (uint32_t)ctx->num_cus)), dim3(256),
+                       0, s, emb, n, frame);
     MDET_HIP(hipMemsetAsync(sums, 0, (size_t)k * E_D * 8, s));
     MDET_HIP(hipMemsetAsync(counts, 0, ((size_t)k + 1) * 4, s));
+    MDET_HIP(hipMemsetAsync(err, 0, 4, s));
     const uint32_t stride = std::max<uint32_t>(1, n / std::max<uint32_t>(1, k * km_sample_per_cluster(k)));
     const uint32_t m = (n - 1) / stride + 1;
+    // 4 row tiles x 4 waves per block where that still gives >= 4 blocks per
+    // CU, else 1 row tile x 8 waves (the centroids split over more waves)
+    auto assign_pass = [&](uint32_t pts, uint32_t str, uint32_t* out, unsigned long long* sm, int mode) {
+        if ((pts + 127) / 128 >= 4u * (uint32_t)ctx->num_cus)
+            hipLaunchKernelGGL((k_km_assign<4, 4>), dim3((pts + 127) / 128), dim3(256), 0, s, emb, pts, str, frame,
+                               cfrag, k, out, counts, sm, mode, err);
+        else
+            hipLaunchKernelGGL((k_km_assign<1, 8>), dim3((pts + 31) / 32), dim3(512), 0, s, emb, pts, str, frame,
+                               cfrag, k, out, counts, sm, mode, err);
+    };
+    const dim3 cf_grid((ntiles * 64 + 255) / 256);
     for (int it = 0; it < km_iters(k); ++it) {
-        hipLaunchKernelGGL(k_km_assign<16>, dim3((m + 63) / 64), dim3(64 * 16), 0, s, emb, m, stride, cent, cnorm, k,
-                           (uint32_t*)nullptr, counts, sums, 2);
-        hipLaunchKernelGGL(k_km_update, dim3((k + 7) / 8), dim3(256), 0, s, sums, counts, k, cent, cnorm);
+        hipLaunchKernelGGL(k_km_cfrag, cf_grid, dim3(256), 0, s, cent, k, frame, cfrag);
+        assign_pass(m, stride, (uint32_t*)nullptr, sums, 2);
+        hipLaunchKernelGGL(k_km_update, dim3((k + 7) / 8), dim3(256), 0, s, sums, counts, k, cent);
     }
-    hipLaunchKernelGGL(k_km_assign<4>, dim3((n + 63) / 64), dim3(64 * 4), 0, s, emb, n, 1u, cent, cnorm, k, assign,
-                       counts, (unsigned long long*)nullptr, 1);
+    hipLaunchKernelGGL(k_km_cfrag, cf_grid, dim3(256), 0, s, cent, k, frame, cfrag);
+    assign_pass(n, 1u, assign, (unsigned long long*)nullptr, 1);
     MDET_HIP(hipGetLastError());
 
     // 2. rows sorted by cluster (stable), cluster offsets
@@ -329,11 +524,13 @@ int ivf_launch(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t
 
     // 4. copies for the host part
     void* hb;
-    MDET_TRY(ctx_host_buf(ctx, "ivf.h_index", ((size_t)k + 1 + (size_t)k * n_probe) * 4, &hb));
+    MDET_TRY(ctx_host_buf(ctx, "ivf.h_index", ((size_t)k + 2 + (size_t)k * n_probe) * 4, &hb));
     uint32_t* h_off = static_cast<uint32_t*>(hb);
     uint32_t* h_probes = h_off + k + 1;
+    uint32_t* h_err = h_probes + (size_t)k * n_probe;
     MDET_HIP(hipMemcpyAsync(h_off, off, ((size_t)k + 1) * 4, hipMemcpyDeviceToHost, s));
     MDET_HIP(hipMemcpyAsync(h_probes, probes, (size_t)k * n_probe * 4, hipMemcpyDeviceToHost, s));
+    MDET_HIP(hipMemcpyAsync(h_err, err, 4, hipMemcpyDeviceToHost, s));
     hipEvent_t ev;
     MDET_TRY(ctx_event(ctx, IVF_EVENT, &ev));
     MDET_HIP(hipEventRecord(ev, s));
@@ -352,6 +549,8 @@ int ivf_finish(mdet_ctx* ctx, IvfIndex* out) {
     const uint32_t k = out->k, n_probe = out->n_probe;
     const uint32_t* offh = out->h_off;
     const uint32_t* probeh = out->h_probes;
+    if (probeh[(size_t)k * n_probe] & 1u)
+        MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges_learned: an embedding row is non-finite or too large to cluster");
     std::vector<uint64_t> cp;
     cp.reserve((size_t)k * n_probe);
     for (uint32_t a = 0; a < k; ++a) {

@@ -640,23 +640,79 @@ __device__ inline void issue_btile(const uint4* __restrict__ frag, uint32_t c0, 
 // neither an atomic nor a global store.
 constexpr int M_STAGE = 128;
 
-__device__ inline void stage_flush(const SurvOut& o, uint32_t shard, uint2* stage, uint32_t& n, int lane) {
+// Chunked survivor output: a flush puts the staged survivors (all of one
+// item) into the next chunk slot of M_STAGE entries of the wave's shard and
+// (item, count) into the slot's meta entry, so sorting the chunks by item
+// lists the survivors item by item (k_chunk_list / k_chunk_compact).  A
+// shard's counter counts chunks and keeps counting past its capacity so the
+// host learns the size needed.
+struct ChunkOut {
+    uint2* buf;               // NSHARD * cap chunks of M_STAGE entries
+    uint2* meta;              // NSHARD * cap (item, count)
+    unsigned long long* cnt;  // NSHARD * SHARD_STRIDE
+    unsigned long long cap;   // chunk slots per shard
+};
+
+__device__ inline void stage_flush(const ChunkOut& o, uint32_t shard, uint2* stage, uint32_t& n, int lane) {
     if (n == 0) return;
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(&o.cnt[(size_t)shard * SHARD_STRIDE], (unsigned long long)n);
-    base = __shfl(base, 0, 64);
-    for (uint32_t k = lane; k < n; k += 64) {
-        const uint2 v = stage[k];
-        shard_put(o, shard, base + k, v.x, v.y);
+    unsigned long long k = 0;
+    if (lane == 0) k = atomicAdd(&o.cnt[(size_t)shard * SHARD_STRIDE], 1ull);
+    k = __shfl(k, 0, 64);
+    if (k < o.cap) {
+        const size_t slot = (size_t)shard * o.cap + k;
+        for (uint32_t e = lane; e < n; e += 64) o.buf[slot * M_STAGE + e] = stage[e];
+        if (lane == 0) o.meta[slot] = make_uint2(blockIdx.x, n);
     }
     n = 0;
+}
+
+// Chunks of shard s (offsets from k_shard_scan over the chunk counters) ->
+// (item, chunk slot) pairs for the sort; survivor total into *total.
+__global__ __launch_bounds__(256) void k_chunk_list(const uint2* __restrict__ meta, unsigned long long cap,
+                                                    const unsigned long long* __restrict__ offsets,
+                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                                    unsigned long long* __restrict__ total) {
+    __shared__ unsigned int bsum;
+    if (threadIdx.x == 0) bsum = 0;
+    __syncthreads();
+    const uint32_t s = blockIdx.x;
+    const unsigned long long b = offsets[s], c = offsets[s + 1] - b;
+    unsigned int sum = 0;
+    for (unsigned long long k = threadIdx.x; k < c; k += blockDim.x) {
+        const uint2 m = meta[(size_t)s * cap + k];
+        key[b + k] = m.x;
+        slot[b + k] = (uint32_t)((size_t)s * cap + k);
+        sum += m.y;
+    }
+    if (sum) atomicAdd(&bsum, sum);
+    __syncthreads();
+    if (threadIdx.x == 0 && bsum) atomicAdd(total, (unsigned long long)bsum);
+}
+
+__global__ __launch_bounds__(256) void k_chunk_counts(const uint2* __restrict__ meta, const uint32_t* __restrict__ slot,
+                                                      uint32_t nchunks, uint32_t* __restrict__ count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nchunks) count[i] = meta[slot[i]].y;
+}
+
+// One wave per chunk (in item order): its entries to their dense offset.
+__global__ __launch_bounds__(256) void k_chunk_compact(const uint2* __restrict__ buf, const uint32_t* __restrict__ slot,
+                                                       const uint32_t* __restrict__ count,
+                                                       const uint64_t* __restrict__ off, uint32_t nchunks,
+                                                       uint2* __restrict__ dense) {
+    const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= nchunks) return;
+    const uint2* src = buf + (size_t)slot[i] * M_STAGE;
+    const uint32_t c = count[i];
+    const uint64_t d = off[i];
+    for (uint32_t e = lane; e < c; e += 64) dense[d + e] = src[e];
 }
 
 template <bool BF16, int KSX>
 __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint4* __restrict__ frag,
                                                            const uint4* __restrict__ frag_a8,
                                                            const uint4* __restrict__ items, uint32_t n,
-                                                           SurvOut out) {
+                                                           ChunkOut out) {
     constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;  // uint4 per B step
     __shared__ uint4 Bl[2][STEP_U4];
     __shared__ uint2 stage_all[4][M_STAGE];
@@ -888,11 +944,20 @@ __global__ __launch_bounds__(256) void k_iperm(const uint32_t* __restrict__ perm
     if (s < n) iperm[perm[s]] = s;
 }
 
+// The candidate list runs item by item, so neighbouring candidates share row
+// buckets: consecutive blocks take 256-candidate pieces FILL_SPREAD pieces
+// apart (piece = (block % FILL_SPREAD) * per + block / FILL_SPREAD), which
+// keeps the blocks in flight on different buckets' counters.
+constexpr uint32_t FILL_SPREAD = 2048;
+
 __global__ __launch_bounds__(256) void k_bucket_fill(const Cand* __restrict__ cand, uint64_t nc,
                                                      const uint32_t* __restrict__ iperm,
                                                      const uint64_t* __restrict__ boff, uint32_t* __restrict__ bcur,
                                                      Ent* __restrict__ ent) {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t pieces = (nc + 255) / 256, per = (pieces + FILL_SPREAD - 1) / FILL_SPREAD;
+    const uint64_t piece = (uint64_t)(blockIdx.x % FILL_SPREAD) * per + blockIdx.x / FILL_SPREAD;
+    if (piece >= pieces) return;  // whole block: the wave collectives below see all lanes
+    const uint64_t k = piece * 256 + threadIdx.x;
     bool act = false;
     Cand c{0, 0, 0.f, 0};
     uint32_t si = 0, sj = 0;
@@ -1714,6 +1779,30 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         uint2 *sbuf = nullptr, *surv = nullptr;
         uint32_t* perm = nullptr;  // sorted position -> contig id (MFMA modes)
         uint64_t ns = 0;
+        // MFMA modes: chunked survivor output, listed item by item afterwards
+        const bool chunked = precision != MDET_EXACT_F32;
+        uint64_t chunk_cap = 64;
+        if (auto it = ctx->bufs.find("pairs.cmeta"); it != ctx->bufs.end())
+            chunk_cap = std::max<uint64_t>(chunk_cap, it->second.bytes / sizeof(uint2) / NSHARD);
+        ChunkOut co{};
+        uint32_t *ckey = nullptr, *cslot = nullptr;
+        uint32_t items_launched = 0;
+        uint64_t nchunks = 0;
+        // a wave flushes once per item it has survivors in, plus once per
+        // M_STAGE survivors; sized for ~128 survivors per contig at first
+        auto chunk_out = [&](uint32_t nitems, int attempt) -> int {
+            if (attempt == 0)
+                chunk_cap = std::max<uint64_t>(
+                    chunk_cap, ((uint64_t)nitems * 4 + (uint64_t)n * 128 / M_STAGE) / NSHARD * 5 / 4 + 16);
+            MDET_TRY(ctx_buf(ctx, "pairs.cbuf", (size_t)NSHARD * chunk_cap * M_STAGE, &co.buf));
+            MDET_TRY(ctx_buf(ctx, "pairs.cmeta", (size_t)NSHARD * chunk_cap, &co.meta));
+            MDET_TRY(ctx_buf(ctx, "pairs.ckey", (size_t)NSHARD * chunk_cap, &ckey));
+            MDET_TRY(ctx_buf(ctx, "pairs.cslot", (size_t)NSHARD * chunk_cap, &cslot));
+            co.cnt = scnt;
+            co.cap = chunk_cap;
+            items_launched = nitems;
+            return MDET_OK;
+        };
         IvfIndex ivf;  // learned mode with probes
         // FP32 rows in the prefilter's order for refinement (MFMA modes)
         float *tnfs = nullptr, *tvs = nullptr;
@@ -1728,11 +1817,11 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             return MDET_OK;
         };
         for (int attempt = 0; attempt < 2; ++attempt) {
-            MDET_TRY(ctx_buf(ctx, "pairs.sbuf", (size_t)NSHARD * shard_cap, &sbuf));
             MDET_HIP(hipMemsetAsync(scnt, 0, (size_t)NSHARD * SHARD_STRIDE * 8, ctx->stream));
             MDET_HIP(hipMemsetAsync(sstat, 0, 8 * 8, ctx->stream));
-            const SurvOut so{sbuf, scnt, (unsigned long long)shard_cap};
             if (precision == MDET_EXACT_F32) {
+                MDET_TRY(ctx_buf(ctx, "pairs.sbuf", (size_t)NSHARD * shard_cap, &sbuf));
+                const SurvOut so{sbuf, scnt, (unsigned long long)shard_cap};
                 float* tnft;
                 MDET_TRY(ctx_buf(ctx, "pairs.tnft", (size_t)NT * n_pad, &tnft));
                 hipLaunchKernelGGL(k_prep_f32t, dim3(n_pad / 64), dim3(256), 0, ctx->stream, tnf_dev, n, n_pad, tnft);
@@ -1771,10 +1860,11 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 uint4* items = ivf.items_dev;
                 uint32_t nitems = ivf.nitems;
                 if (!n_probe) MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (nitems)
                     hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
-                                       frag_a, items, n, so);
+                                       frag_a, items, n, co);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
                 // |d~ - d| <= |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit
@@ -1795,36 +1885,71 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 uint4* items;
                 uint32_t nitems;
                 MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (bf)
                     hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
-                                       frag_a8, items, n, so);
+                                       frag_a8, items, n, co);
                 else
                     hipLaunchKernelGGL((k_prefilter_mfma<false, KS>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
-                                       frag_a8, items, n, so);
+                                       frag_a8, items, n, co);
             }
             MDET_HIP(hipGetLastError());
             if (attempt == 0) MDET_TRY(tm.record(2));
-            hipLaunchKernelGGL(k_shard_scan, dim3(1), dim3(1024), 0, ctx->stream, scnt, (unsigned long long)shard_cap,
+            const uint64_t cap_now = chunked ? chunk_cap : shard_cap;
+            hipLaunchKernelGGL(k_shard_scan, dim3(1), dim3(1024), 0, ctx->stream, scnt, (unsigned long long)cap_now,
                                soff, sstat);
-            unsigned long long st2[2];
+            if (chunked)
+                hipLaunchKernelGGL(k_chunk_list, dim3(NSHARD), dim3(256), 0, ctx->stream, co.meta,
+                                   (unsigned long long)chunk_cap, soff, ckey, cslot, sstat + 2);
+            unsigned long long st2[3];
             {
                 void* st_host;
                 MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
-                MDET_HIP(hipMemcpyAsync(st_host, sstat, 16, hipMemcpyDeviceToHost, ctx->stream));
+                MDET_HIP(hipMemcpyAsync(st_host, sstat, 24, hipMemcpyDeviceToHost, ctx->stream));
                 MDET_HIP(hipStreamSynchronize(ctx->stream));
-                memcpy(st2, st_host, 16);
+                memcpy(st2, st_host, 24);
             }
-            if (st2[1] <= shard_cap) {
-                ns = st2[0];
+            // st2: [0] entries (chunks if chunked) kept, [1] largest raw shard count, [2] chunked survivors
+            if (st2[1] <= cap_now) {
+                ns = chunked ? st2[2] : st2[0];
+                nchunks = chunked ? st2[0] : 0;
                 break;
             }
             if (attempt == 1) MDET_FAIL(MDET_E_STATE, "mdet_tnf_edges: survivor count grew between passes");
-            shard_cap = st2[1] + st2[1] / 4 + 64;
+            if (chunked)
+                chunk_cap = st2[1] + st2[1] / 4 + 16;
+            else
+                shard_cap = st2[1] + st2[1] / 4 + 64;
         }
         MDET_TRY(ctx_buf(ctx, "pairs.surv", std::max<uint64_t>(ns, 1), &surv));
-        hipLaunchKernelGGL(k_compact, dim3(NSHARD), dim3(256), 0, ctx->stream, sbuf, (unsigned long long)shard_cap,
-                           soff, surv);
+        if (chunked && nchunks) {
+            // chunks in item order (stable within an item), then their entries packed
+            if (nchunks > 0x7fffffffull) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: too many survivor chunks");
+            uint32_t *ckey2, *cslot2, *ccnt;
+            uint64_t* coff;
+            MDET_TRY(ctx_buf(ctx, "pairs.ckey2", nchunks, &ckey2));
+            MDET_TRY(ctx_buf(ctx, "pairs.cslot2", nchunks, &cslot2));
+            MDET_TRY(ctx_buf(ctx, "pairs.ccnt", nchunks + 1, &ccnt));  // the scan reads one past the end
+            MDET_TRY(ctx_buf(ctx, "pairs.coff", nchunks + 1, &coff));
+            int end_bit = 1;
+            while (end_bit < 32 && (1ull << end_bit) < items_launched) ++end_bit;
+            size_t tmp = 0;
+            MDET_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ckey, ckey2, cslot, cslot2, (int)nchunks, 0,
+                                                        end_bit, ctx->stream));
+            void* t;
+            MDET_TRY(ctx_buf(ctx, "cub.sorttmp", tmp, &t));
+            MDET_HIP(hipcub::DeviceRadixSort::SortPairs(t, tmp, ckey, ckey2, cslot, cslot2, (int)nchunks, 0, end_bit,
+                                                        ctx->stream));
+            hipLaunchKernelGGL(k_chunk_counts, dim3((unsigned)((nchunks + 255) / 256)), dim3(256), 0, ctx->stream,
+                               co.meta, cslot2, (uint32_t)nchunks, ccnt);
+            MDET_TRY(exclusive_scan_u32_to_u64(ctx, ccnt, coff, (uint32_t)nchunks));
+            hipLaunchKernelGGL(k_chunk_compact, dim3((unsigned)((nchunks + 3) / 4)), dim3(256), 0, ctx->stream, co.buf,
+                               cslot2, ccnt, coff, (uint32_t)nchunks, surv);
+        } else if (!chunked) {
+            hipLaunchKernelGGL(k_compact, dim3(NSHARD), dim3(256), 0, ctx->stream, sbuf,
+                               (unsigned long long)shard_cap, soff, surv);
+        }
         MDET_HIP(hipGetLastError());
         st.survivors = ns;
         st.pairs_evaluated = (precision == MDET_LEARNED && n_probe) ? ivf.pairs : st.pairs_total;
@@ -1893,7 +2018,10 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 MDET_TRY(ctx_buf(ctx, "sel.iperm", (size_t)n, &iperm));
                 hipLaunchKernelGGL(k_iperm, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, perm, n, iperm);
             }
-            const unsigned cblocks = (unsigned)((nc + 255) / 256);
+            const uint64_t pieces = (nc + 255) / 256;
+            const unsigned cblocks = (unsigned)(pieces <= FILL_SPREAD
+                                                    ? pieces
+                                                    : (uint64_t)FILL_SPREAD * ((pieces + FILL_SPREAD - 1) / FILL_SPREAD));
             MDET_TRY(exclusive_scan_u32_to_u64(ctx, bcnt, boff, nb));
             hipLaunchKernelGGL(k_bucket_fill, dim3(cblocks), dim3(256), 0, ctx->stream, cand, nc, iperm, boff, bcur, ent);
             hipLaunchKernelGGL(k_bucket_select, dim3(nb), dim3(S_THREADS), 0, ctx->stream, boff, ent, adj, n, max_edges,

@@ -17,6 +17,28 @@ import numpy as np
 from . import BF16_REFINE, Ctx, DevArray, Mlp
 
 
+def _pair_keys(rec: np.ndarray, n: int, seed: int):
+    """Unordered pair key min * n + max and a splitmix64 hash of it."""
+    i, j = rec["i"].astype(np.uint64), rec["j"].astype(np.uint64)
+    key = np.minimum(i, j) * np.uint64(n) + np.maximum(i, j)
+    z = key + np.uint64((0x9E3779B97F4A7C15 * (seed + 1)) & 0xFFFFFFFFFFFFFFFF)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return key, z ^ (z >> np.uint64(31))
+
+
+def _sample_pairs(rec: np.ndarray, m: int, n: int, seed: int, rng) -> np.ndarray:
+    """m records of rec chosen independently of rec's order (which follows the
+    device's internal survivor order): the m smallest pair hashes, in key
+    order; with replacement from the key-sorted records if m > len(rec)."""
+    key, h = _pair_keys(rec, n, seed)
+    if m >= len(rec):
+        srt = rec[np.argsort(key, kind="stable")]
+        return srt if m == len(rec) else srt[rng.choice(len(srt), m, replace=True)]
+    take = np.argpartition(h, m - 1)[:m] if m else np.zeros(0, np.int64)
+    return rec[take[np.argsort(key[take], kind="stable")]]
+
+
 def training_set(ctx: Ctx, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int,
                  n_pairs: int, seed: int = 0):
     """(pi, pj, y) with y in {0,1}, positives : negatives = 1 : 4."""
@@ -26,10 +48,9 @@ def training_set(ctx: Ctx, tnf: DevArray, logsize: DevArray, n: int, cutoff: flo
     is_pos = (rec["flags"] & 2) != 0
     pos, hard = rec[is_pos], rec[~is_pos]
     n_pos = min(len(pos), max(1, n_pairs // 5))
-    take = rng.choice(len(pos), n_pos, replace=len(pos) < n_pos)
-    P = pos[take]
+    P = _sample_pairs(pos, n_pos, n, seed, rng)
     n_hard = 2 * n_pos if len(hard) else 0
-    H = hard[rng.choice(len(hard), n_hard, replace=len(hard) < n_hard)] if n_hard else hard[:0]
+    H = _sample_pairs(hard, n_hard, n, seed + 1, rng) if n_hard else hard[:0]
     n_rand = 4 * n_pos - n_hard
     ri = rng.integers(0, n, n_rand).astype(np.uint64)
     rj = rng.integers(0, n, n_rand).astype(np.uint64)
