@@ -135,7 +135,7 @@ def cpu_count_baseline(w, cfg_name, budget_contigs=30000):
                       f"openmp/TNF.cpp counts), its own timer around the TNF loop: {secs:.3f} s"}
 
 
-def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9995):
+def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999):
     """Train the filter on pairs the exact path labels, then time the learned
     edge list (cluster index over the embeddings, n_probe nearest clusters)
     and measure its recall against the exhaustive edge set."""

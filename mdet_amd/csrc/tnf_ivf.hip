@@ -551,33 +551,65 @@ int ivf_finish(mdet_ctx* ctx, IvfIndex* out) {
     const uint32_t* probeh = out->h_probes;
     if (probeh[(size_t)k * n_probe] & 1u)
         MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges_learned: an embedding row is non-finite or too large to cluster");
-    std::vector<uint64_t> cp;
+    auto usable = [&](uint32_t b) { return b < k && offh[b + 1] > offh[b]; };
+    // distinct unordered cluster pairs {a, b} with b in a's probes or a in
+    // b's, each once: emitted by a for b >= a, and for b < a only when a is
+    // not among b's probes (then b emits it).  Bitmap of (b, a) listings.
+    thread_local std::vector<uint64_t> listed;  // all zero between calls
+    if (listed.size() < ((size_t)k * k + 63) / 64) listed.assign(((size_t)k * k + 63) / 64, 0ull);
+    for (uint32_t b = 0; b < k; ++b)
+        for (uint32_t p = 0; p < n_probe; ++p) {
+            const uint32_t a = probeh[(size_t)b * n_probe + p];
+            if (a < k) listed[((size_t)b * k + a) >> 6] |= 1ull << (((size_t)b * k + a) & 63);
+        }
+    auto is_listed = [&](uint32_t b, uint32_t a) {
+        return (listed[((size_t)b * k + a) >> 6] >> (((size_t)b * k + a) & 63)) & 1ull;
+    };
+    std::vector<uint64_t> cp;  // (rows cluster << 32) | column cluster, rows <= columns
     cp.reserve((size_t)k * n_probe);
+    uint64_t nitems = 0;
     for (uint32_t a = 0; a < k; ++a) {
-        if (offh[a + 1] == offh[a]) continue;
+        if (!usable(a)) continue;
         for (uint32_t p = 0; p < n_probe; ++p) {
             const uint32_t b = probeh[(size_t)a * n_probe + p];
-            if (b == 0xffffffffu || b >= k || offh[b + 1] == offh[b]) continue;
-            cp.push_back(((uint64_t)std::min(a, b) << 32) | std::max(a, b));
+            if (!usable(b) || (b < a && is_listed(b, a))) continue;
+            const uint32_t lo = std::min(a, b), hi = std::max(a, b);
+            cp.push_back(((uint64_t)lo << 32) | hi);
+            // items: the rows of cluster lo in pieces ending at 32-aligned
+            // ITEM_ROWS boundaries (a single-row diagonal piece has no pair)
+            for (uint32_t rs = offh[lo], r1 = offh[lo + 1]; rs < r1;) {
+                const uint32_t re = std::min<uint32_t>((rs & ~31u) + ITEM_ROWS, r1);
+                nitems += (lo != hi || rs + 1 < r1) ? 1 : 0;
+                rs = re;
+            }
         }
     }
-    std::sort(cp.begin(), cp.end());
-    cp.erase(std::unique(cp.begin(), cp.end()), cp.end());
-    std::vector<uint4> items;
-    items.reserve(cp.size() * 2);
-    uint64_t pairs = 0;
+    for (uint32_t b = 0; b < k; ++b)
+        for (uint32_t p = 0; p < n_probe; ++p) {
+            const uint32_t a = probeh[(size_t)b * n_probe + p];
+            if (a < k) listed[((size_t)b * k + a) >> 6] = 0ull;
+        }
+    if (nitems > 0x7fffffffull) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges_learned: too many cluster items");
+    uint4* items = nullptr;
+    if (nitems) {
+        // the previous call's upload from this buffer finished before that call returned
+        void* hb;
+        MDET_TRY(ctx_host_buf(ctx, "ivf.h_items", nitems * sizeof(uint4), &hb));
+        items = static_cast<uint4*>(hb);
+    }
+    uint64_t pairs = 0, w = 0;
     for (uint64_t key : cp) {
         const uint32_t a = (uint32_t)(key >> 32), b = (uint32_t)key;
         const uint32_t a0 = offh[a], a1 = offh[a + 1], b0 = offh[b], b1 = offh[b + 1];
         for (uint32_t rs = a0; rs < a1;) {
             const uint32_t re = std::min<uint32_t>((rs & ~31u) + ITEM_ROWS, a1);
             if (a == b) {
-                if (rs + 1 < a1) items.push_back(make_uint4(rs, re, rs, a1 | ITEM_TRI));
+                if (rs + 1 < a1) items[w++] = make_uint4(rs, re, rs, a1 | ITEM_TRI);
                 // pairs i < j, i in [rs, re), j in [rs, a1): sum of (a1 - 1 - i)
                 const uint64_t cnt = re - rs;
                 pairs += cnt * (a1 - 1) - (cnt * (2 * (uint64_t)rs + cnt - 1)) / 2;
             } else {
-                items.push_back(make_uint4(rs, re, b0, b1));
+                items[w++] = make_uint4(rs, re, b0, b1);
                 pairs += (uint64_t)(re - rs) * (b1 - b0);
             }
             rs = re;
@@ -585,15 +617,10 @@ int ivf_finish(mdet_ctx* ctx, IvfIndex* out) {
     }
     out->pairs = pairs;
     out->probe_pairs = (uint32_t)cp.size();
-    out->nitems = (uint32_t)items.size();
-    MDET_TRY(ctx_buf(ctx, "ivf.items", std::max<size_t>(items.size(), 1), &out->items_dev));
-    if (!items.empty()) {
-        // the previous call's upload from this buffer finished before that call returned
-        void* hb;
-        MDET_TRY(ctx_host_buf(ctx, "ivf.h_items", items.size() * sizeof(uint4), &hb));
-        memcpy(hb, items.data(), items.size() * sizeof(uint4));
-        MDET_HIP(hipMemcpyAsync(out->items_dev, hb, items.size() * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
-    }
+    out->nitems = (uint32_t)nitems;
+    MDET_TRY(ctx_buf(ctx, "ivf.items", std::max<size_t>(nitems, 1), &out->items_dev));
+    if (nitems)
+        MDET_HIP(hipMemcpyAsync(out->items_dev, items, nitems * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
     return MDET_OK;
 }
 

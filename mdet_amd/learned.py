@@ -102,7 +102,7 @@ def embeddings(ctx: Ctx, mlp: Mlp, tnf: DevArray, n: int) -> np.ndarray:
     return out
 
 
-def calibrate_radius2(emb: np.ndarray, pi: np.ndarray, pj: np.ndarray, quantile: float = 0.9995,
+def calibrate_radius2(emb: np.ndarray, pi: np.ndarray, pj: np.ndarray, quantile: float = 0.9999,
                       slack: float = 1.05) -> float:
     """Squared embedding radius holding the given fraction of the positive pairs."""
     d = emb[pi.astype(np.int64)] - emb[pj.astype(np.int64)]
