@@ -135,7 +135,7 @@ def cpu_count_baseline(w, cfg_name, budget_contigs=30000):
                       f"openmp/TNF.cpp counts), its own timer around the TNF loop: {secs:.3f} s"}
 
 
-def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999):
+def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, autotune=True):
     """Train the filter on pairs the exact path labels, then time the learned
     edge list (cluster index over the embeddings, n_probe nearest clusters)
     and measure its recall against the exhaustive edge set."""
@@ -151,6 +151,8 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999):
     emb = L.embeddings(ctx, mlp, w.tnf, n)
     r2 = L.calibrate_radius2(emb, pos["i"], pos["j"], quantile=quantile)
     kw = dict(mlp=mlp, radius2=r2, n_probe=n_probe)
+    tune = ctx.autotune(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, reps=2, **kw) if autotune \
+        else ctx.tune_get(M.LEARNED)
     for _ in range(2):
         ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, **kw)
     ctx.sync()
@@ -173,7 +175,7 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999):
             "survivors_exact_evaluated": stats[-1]["survivors"], "edges": int(len(f)),
             "edge_recall": float(hit.mean()) if len(exact) else 1.0,
             "edges_not_in_exhaustive_set": int((~np.isin(got, exact)).sum()),
-            "radius2": r2, "radius_quantile": quantile,
+            "radius2": r2, "radius_quantile": quantile, "tune": tune,
             "training": {"pairs": int(len(pi)), "positives": int(y.sum()), "steps": train_steps, "batch": 512,
                          "seconds": train_s, "ms_per_step": 1e3 * train_s / max(1, train_steps),
                          "set_build_s": set_s, "loss_curve": [(int(a), float(b)) for a, b in curve]}}
@@ -237,6 +239,8 @@ def main():
     ap.add_argument("--modes", default="exact_f32,f16_refine")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the BF16 and learned paths at the default launch shapes")
     ap.add_argument("--n-probe", type=int, default=0,
                     help="clusters probed per cluster by the learned filter (0: 16 up to 262,144 contigs, else 32)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -263,7 +267,13 @@ def main():
     for a in (w.packed, w.runs, w.base_off):
         a.free()
 
-    # headline: BF16 MFMA prefilter + FP32 refine
+    # headline: BF16 MFMA prefilter + FP32 refine, at the launch shape the autotuner picks for this input
+    if args.no_autotune:
+        tune = ctx.tune_get(M.BF16_REFINE)
+    else:
+        log("autotuning bf16_refine")
+        tune = ctx.autotune(w.tnf, w.logsize, w.n, CUTOFF, MAX_EDGES, M.BF16_REFINE, reps=2)
+        log(f"  {tune}")
     log("timing bf16_refine")
     sec, stats = run_mode(ctx, w, M.BF16_REFINE, args.steps, args.warmup)
     pf_ms = float(np.mean([s["ms_prefilter"] for s in stats]))
@@ -271,7 +281,7 @@ def main():
                              "stage_ms": {k: float(np.mean([s[k] for s in stats]))
                                           for k in ("ms_prefilter", "ms_refine", "ms_select", "ms_total")},
                              "survivors": stats[-1]["survivors"], "candidates": stats[-1]["candidates"],
-                             "adjudicated_fp64": stats[-1]["adjudicated"]}}
+                             "adjudicated_fp64": stats[-1]["adjudicated"], "tune": tune}}
     fb, tb, sb, stb = fetch_edges(ctx, w, M.BF16_REFINE)
     modes["bf16_refine"]["edges"] = int(len(fb))
 
@@ -294,7 +304,7 @@ def main():
     if not args.no_learned:
         log(f"learned filter: {args.train_steps} training steps")
         n_probe = args.n_probe or (16 if w.n <= 262144 else 32)
-        lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, n_probe)
+        lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, n_probe, autotune=not args.no_autotune)
         lm["speedup_vs_bf16_refine"] = lm["pairs_per_s"] / modes["bf16_refine"]["pairs_per_s"]
         modes["learned"] = lm
 

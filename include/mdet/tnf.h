@@ -236,6 +236,41 @@ int mdet_tnf_edges_learned(mdet_ctx *ctx, mdet_mlp *mlp, const float *tnf_dev, c
                            double cutoff, uint32_t max_edges, float radius2, uint32_t n_clusters, uint32_t n_probe,
                            mdet_edges *out, mdet_pair_stats *stats);
 
+/* ------------------------------------------------------------ autotuner -- */
+/* No reference counterpart: the reference's only tuning is its fixed TILE
+ * heuristic (cuda/Cuda2.cu:868-874) and the --ct/--cb/--cs launch flags
+ * (cuda/Cuda.cu:1869-1871).
+ * Launch shapes of the pair stage, one set per precision mode, held by the
+ * context and used by every later edge-list call on it.  No shape changes
+ * the edge list. */
+typedef struct mdet_tune {
+    uint32_t seg_cols;      /* MFMA Gram modes: prefilter item column span, a
+                             * multiple of 256 in [256, 65536] (default 1024) */
+    uint32_t persistent;    /* MFMA modes: 0 one prefilter block per item
+                             * (default), 1 resident blocks take items from a
+                             * counter */
+    uint32_t refine_blocks; /* refinement blocks per CU, 1..64 (default 32) */
+    uint32_t reserved;      /* 0 */
+    double ms_prefilter;    /* autotuner: median stage times at the chosen */
+    double ms_refine;       /* shape (0 when never tuned) */
+} mdet_tune;
+int mdet_tune_get(mdet_ctx *ctx, int precision, mdet_tune *out);
+int mdet_tune_set(mdet_ctx *ctx, int precision, const mdet_tune *in);
+/* Times the candidate shapes on this input (each the median of reps calls of
+ * mdet_tnf_edges; reps 1..15): prefilter column spans 512/1024/2048/4096 x
+ * per-item or persistent blocks (for the Gram modes; the exact mode has no
+ * prefilter shape) by the prefilter stage time, then refine blocks per CU
+ * 8/16/32/64 by the refine stage time.  Keeps the fastest in ctx and returns
+ * it (out may be NULL); the last call's edge list stays in ctx as after
+ * mdet_tnf_edges.  ~(8 + 4) x reps + 1 edge-list calls. */
+int mdet_tnf_autotune(mdet_ctx *ctx, const float *tnf_dev, const double *logsize_dev, uint32_t n, double cutoff,
+                      uint32_t max_edges, int precision, uint32_t reps, mdet_tune *out);
+/* The same for the learned mode (per-item or persistent prefilter blocks,
+ * refine blocks per CU), with mdet_tnf_edges_learned's arguments. */
+int mdet_tnf_autotune_learned(mdet_ctx *ctx, mdet_mlp *mlp, const float *tnf_dev, const double *logsize_dev,
+                              uint32_t n, double cutoff, uint32_t max_edges, float radius2, uint32_t n_clusters,
+                              uint32_t n_probe, uint32_t reps, mdet_tune *out);
+
 /*
  * Sampled row maxima for the automatic cutoff (pTNF = 0): for each sampled
  * position p < n_sample, max_out_dev[p] = 1 - 1/(1 + exp(m)) with m the

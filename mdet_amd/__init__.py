@@ -44,6 +44,14 @@ class PairStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Tune(C.Structure):
+    _fields_ = [("seg_cols", C.c_uint32), ("persistent", C.c_uint32), ("refine_blocks", C.c_uint32),
+                ("reserved", C.c_uint32), ("ms_prefilter", C.c_double), ("ms_refine", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 _lib = None
 
 # name -> (restype, argtypes); every function returns an int status unless noted
@@ -91,6 +99,13 @@ _SIGS = {
     "mdet_tnf_edges_learned": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double,
                                          C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(Edges),
                                          C.POINTER(PairStats)]),
+    "mdet_tune_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tune)]),
+    "mdet_tune_set": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tune)]),
+    "mdet_tnf_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, C.c_int,
+                                    C.c_uint32, C.POINTER(Tune)]),
+    "mdet_tnf_autotune_learned": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_double,
+                                            C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.POINTER(Tune)]),
     "mdet_tnf_rowmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                   C.c_void_p]),
     "mdet_tnf_preprob": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
@@ -345,6 +360,30 @@ class Ctx:
         _check(lib().mdet_tnf_count_ascii(self.h, C.c_void_p(seqs.ptr), C.c_void_p(index.ptr), n,
                                           C.c_void_p(counts.ptr if counts is not None else None),
                                           C.c_void_p(tnf.ptr)), "mdet_tnf_count_ascii")
+
+    def tune_get(self, precision: int) -> dict:
+        t = Tune()
+        _check(lib().mdet_tune_get(self.h, precision, C.byref(t)), "mdet_tune_get")
+        return t.as_dict()
+
+    def tune_set(self, precision: int, seg_cols: int = 1024, persistent: int = 0, refine_blocks: int = 32):
+        t = Tune(seg_cols, persistent, refine_blocks, 0, 0.0, 0.0)
+        _check(lib().mdet_tune_set(self.h, precision, C.byref(t)), "mdet_tune_set")
+
+    def autotune(self, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int, precision: int,
+                 reps: int = 3, mlp: "Mlp | None" = None, radius2: float = 0.0, n_probe: int = 0,
+                 n_clusters: int = 0) -> dict:
+        """Times the pair stage's launch shapes on this input and keeps the fastest
+        for the precision mode (see mdet_tnf_autotune)."""
+        t = Tune()
+        if precision == LEARNED:
+            _check(lib().mdet_tnf_autotune_learned(self.h, mlp.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n,
+                                                   cutoff, max_edges, radius2, n_clusters, n_probe, reps,
+                                                   C.byref(t)), "mdet_tnf_autotune_learned")
+        else:
+            _check(lib().mdet_tnf_autotune(self.h, C.c_void_p(tnf.ptr), C.c_void_p(logsize.ptr), n, cutoff,
+                                           max_edges, precision, reps, C.byref(t)), "mdet_tnf_autotune")
+        return t.as_dict()
 
     def tnf_edges(self, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int,
                   precision: int = EXACT_F32, fetch: bool = True, mlp: "Mlp | None" = None, radius2: float = 0.0,

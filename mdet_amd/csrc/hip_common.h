@@ -45,9 +45,19 @@ struct mdet_ctx {
     double radius_key[3] = {0, 0, 0};
     double radius = 0;
     bool radius_valid = false;
-    // cached MFMA work items for a padded row count
+    // cached MFMA work items for a padded row count and column span
     uint32_t items_npad = 0;
+    uint32_t items_seg = 0;
     uint32_t nitems = 0;
+    // pair-stage launch shapes per precision mode (set by hand or by the
+    // autotuner): prefilter item column span, persistent prefilter blocks,
+    // refine blocks per CU; the autotuner's stage times at that shape
+    struct Tune {
+        uint32_t seg_cols = 1024;
+        uint32_t persistent = 0;
+        uint32_t refine_blocks = 32;
+        double ms_prefilter = 0, ms_refine = 0;
+    } tune[4];
     uint64_t last_pairs = 0;  // refine records of the last edge-list call
 };
 

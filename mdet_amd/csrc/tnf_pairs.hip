@@ -600,10 +600,11 @@ __global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__
 // "acc > 0": one compare, counted per 32x32 tile with a wave ballot.  Only
 // steps with survivors enter the extraction pass, whose cost scales with the
 // number of (tile, accumulator) slots that hold survivors.
-// Column segment per item: at 200k contigs 1024 columns ran the step 3% faster
-// than 2048 (finer scheduling of the survivor-heavy diagonal items) with
-// shorter survivor runs for refinement; 512 and 768 were no faster overall.
-constexpr int M_ROWS = 256, M_STEP = 128, M_SEG = 1024;
+// Column span per item (mdet_tune.seg_cols, default 1024, chosen per input by
+// the autotuner): at 200k contigs 1024 columns ran the step 3% faster than
+// 2048 (finer scheduling of the survivor-heavy diagonal items) with shorter
+// survivor runs for refinement; 512 and 768 were no faster overall.
+constexpr int M_ROWS = 256, M_STEP = 128;
 constexpr int M_MIN_BLOCKS = 2;  // resident blocks per CU the registers are sized for
 // Prefilter item: (r_lo, r_hi, c_lo, c_hi | ITEM_TRI) = pairs of row positions
 // [r_lo, r_hi) and column positions [c_lo, c_hi), only i < j if ITEM_TRI;
@@ -653,7 +654,8 @@ struct ChunkOut {
     unsigned long long cap;   // chunk slots per shard
 };
 
-__device__ inline void stage_flush(const ChunkOut& o, uint32_t shard, uint2* stage, uint32_t& n, int lane) {
+__device__ inline void stage_flush(const ChunkOut& o, uint32_t shard, uint32_t item, uint2* stage, uint32_t& n,
+                                   int lane) {
     if (n == 0) return;
     unsigned long long k = 0;
     if (lane == 0) k = atomicAdd(&o.cnt[(size_t)shard * SHARD_STRIDE], 1ull);
@@ -661,7 +663,7 @@ __device__ inline void stage_flush(const ChunkOut& o, uint32_t shard, uint2* sta
     if (k < o.cap) {
         const size_t slot = (size_t)shard * o.cap + k;
         for (uint32_t e = lane; e < n; e += 64) o.buf[slot * M_STAGE + e] = stage[e];
-        if (lane == 0) o.meta[slot] = make_uint2(blockIdx.x, n);
+        if (lane == 0) o.meta[slot] = make_uint2(item, n);
     }
     n = 0;
 }
@@ -709,14 +711,10 @@ __global__ __launch_bounds__(256) void k_chunk_compact(const uint2* __restrict__
 }
 
 template <bool BF16, int KSX>
-__global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint4* __restrict__ frag,
-                                                           const uint4* __restrict__ frag_a8,
-                                                           const uint4* __restrict__ items, uint32_t n,
-                                                           ChunkOut out) {
-    constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;  // uint4 per B step
-    __shared__ uint4 Bl[2][STEP_U4];
-    __shared__ uint2 stage_all[4][M_STAGE];
-    const uint4 it = items[blockIdx.x];
+__device__ __forceinline__ void prefilter_item(const uint4* __restrict__ frag, const uint4* __restrict__ frag_a8,
+                                               const uint4 it, const uint32_t item,
+                                               uint4 (*Bl)[M_STEP / 32 * KSX * 64], uint2 (*stage_all)[M_STAGE],
+                                               const ChunkOut& out) {
     const uint32_t r_lo = it.x, r_hi = it.y, c_lo = it.z, c_hi = it.w & ~ITEM_TRI;
     const bool tri = (it.w & ITEM_TRI) != 0;
     const uint32_t c_begin = c_lo & ~31u;
@@ -806,7 +804,7 @@ __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint
                         const uint64_t m = __ballot(h);
                         if (m) {
                             const uint32_t c = (uint32_t)__popcll(m);
-                            if (nstage + c > M_STAGE) stage_flush(out, shard, stage, nstage, lane);
+                            if (nstage + c > M_STAGE) stage_flush(out, shard, item, stage, nstage, lane);
                             if (h) stage[nstage + (uint32_t)__popcll(m & below)] = make_uint2(i, j);
                             nstage += c;
                         }
@@ -814,7 +812,34 @@ __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint
                 }
         }
     }
-    stage_flush(out, shard, stage, nstage, lane);
+    stage_flush(out, shard, item, stage, nstage, lane);
+}
+
+// One block per item (item_ctr == nullptr: item = blockIdx.x), or persistent
+// blocks that take items from *item_ctr until nitems are handed out.
+template <bool BF16, int KSX>
+__global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint4* __restrict__ frag,
+                                                           const uint4* __restrict__ frag_a8,
+                                                           const uint4* __restrict__ items, uint32_t nitems,
+                                                           uint32_t* __restrict__ item_ctr, ChunkOut out) {
+    constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;  // uint4 per B step
+    __shared__ uint4 Bl[2][STEP_U4];
+    __shared__ uint2 stage_all[4][M_STAGE];
+    __shared__ uint32_t s_item;
+    uint32_t item = blockIdx.x;
+    if (item_ctr) {
+        if (threadIdx.x == 0) s_item = atomicAdd(item_ctr, 1u);
+        __syncthreads();
+        item = s_item;
+    }
+    while (item < nitems) {
+        prefilter_item<BF16, KSX>(frag, frag_a8, items[item], item, Bl, stage_all, out);
+        if (!item_ctr) break;
+        __syncthreads();  // every wave is done with Bl and s_item
+        if (threadIdx.x == 0) s_item = atomicAdd(item_ctr, 1u);
+        __syncthreads();
+        item = s_item;
+    }
 }
 
 // ---------------------------------------------------------------- refine --
@@ -1657,13 +1682,13 @@ int sort_by_composition(mdet_ctx* ctx, const float* tnf_dev, uint32_t n, uint32_
 
 // Prefilter items for the MFMA kernel: (row block, column begin, column end),
 // row-block major, each row block's columns from its diagonal in segments of
-// M_SEG.  The survivor-heavy diagonal item of every row block is then one of
-// ~n/M_SEG items in flight, so extraction work interleaves with MFMA-only
-// blocks on every CU.  (A segment-major order with an XCD-aware block map cut
+// seg columns.  The survivor-heavy diagonal item of every row block is then
+// one of ~n/seg items in flight, so extraction work interleaves with
+// MFMA-only blocks on every CU.  (A segment-major order with an XCD-aware block map cut
 // the prefilter's L2-miss traffic from 19 GB to 2.9 GB at 200k contigs but ran
 // 8% slower: the kernel is bound by instruction issue, not by those fetches.)
-int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nitems) {
-    if (ctx->items_npad == n_pad && ctx->nitems) {
+int build_items(mdet_ctx* ctx, uint32_t n_pad, uint32_t seg, uint4** items_dev, uint32_t* nitems) {
+    if (ctx->items_npad == n_pad && ctx->items_seg == seg && ctx->nitems) {
         auto it = ctx->bufs.find("pairs.items");
         if (it != ctx->bufs.end()) {
             *items_dev = static_cast<uint4*>(it->second.p);
@@ -1674,14 +1699,15 @@ int build_items(mdet_ctx* ctx, uint32_t n_pad, uint4** items_dev, uint32_t* nite
     std::vector<uint4> items;
     const uint32_t nrb = n_pad / M_ROWS;
     for (uint32_t rb = 0; rb < nrb; ++rb)
-        for (uint32_t c = rb * M_ROWS; c < n_pad; c += M_SEG)
-            items.push_back(make_uint4(rb * M_ROWS, rb * M_ROWS + M_ROWS, c, std::min<uint32_t>(c + M_SEG, n_pad) | ITEM_TRI));
+        for (uint32_t c = rb * M_ROWS; c < n_pad; c += seg)
+            items.push_back(make_uint4(rb * M_ROWS, rb * M_ROWS + M_ROWS, c, std::min<uint32_t>(c + seg, n_pad) | ITEM_TRI));
     *nitems = (uint32_t)items.size();
     MDET_TRY(ctx_buf(ctx, "pairs.items", items.size(), items_dev));
     MDET_HIP(hipMemcpyAsync(*items_dev, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice,
                             ctx->stream));
     MDET_HIP(hipStreamSynchronize(ctx->stream));
     ctx->items_npad = n_pad;
+    ctx->items_seg = seg;
     ctx->nitems = *nitems;
     return MDET_OK;
 }
@@ -1790,7 +1816,17 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         uint64_t nchunks = 0;
         // a wave flushes once per item it has survivors in, plus once per
         // M_STAGE survivors; sized for ~128 survivors per contig at first
+        const mdet_ctx::Tune tune = ctx->tune[precision];
+        // persistent prefilter: M_MIN_BLOCKS resident blocks per CU pull items from a counter
+        uint32_t* pf_ctr = nullptr;
+        auto pf_grid = [&](uint32_t nitems) {
+            return dim3(pf_ctr ? std::min<uint32_t>(nitems, (uint32_t)ctx->num_cus * M_MIN_BLOCKS) : nitems);
+        };
         auto chunk_out = [&](uint32_t nitems, int attempt) -> int {
+            if (tune.persistent) {
+                MDET_TRY(ctx_buf(ctx, "pairs.item_ctr", 1, &pf_ctr));
+                MDET_HIP(hipMemsetAsync(pf_ctr, 0, 4, ctx->stream));
+            }
             if (attempt == 0)
                 chunk_cap = std::max<uint64_t>(
                     chunk_cap, ((uint64_t)nitems * 4 + (uint64_t)n * 128 / M_STAGE) / NSHARD * 5 / 4 + 16);
@@ -1859,12 +1895,12 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 }
                 uint4* items = ivf.items_dev;
                 uint32_t nitems = ivf.nitems;
-                if (!n_probe) MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                if (!n_probe) MDET_TRY(build_items(ctx, n_pad, tune.seg_cols, &items, &nitems));
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (nitems)
-                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
-                                       frag_a, items, n, co);
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), pf_grid(nitems), dim3(256), 0, ctx->stream,
+                                       frag, frag_a, items, nitems, pf_ctr, co);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
                 // |d~ - d| <= |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit
@@ -1884,15 +1920,15 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                                        tnf_dev, perm, n, n_pad, R2, frag, frag_a8);
                 uint4* items;
                 uint32_t nitems;
-                MDET_TRY(build_items(ctx, n_pad, &items, &nitems));
+                MDET_TRY(build_items(ctx, n_pad, tune.seg_cols, &items, &nitems));
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (bf)
-                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
-                                       frag_a8, items, n, co);
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), pf_grid(nitems), dim3(256), 0, ctx->stream, frag,
+                                       frag_a8, items, nitems, pf_ctr, co);
                 else
-                    hipLaunchKernelGGL((k_prefilter_mfma<false, KS>), dim3(nitems), dim3(256), 0, ctx->stream, frag,
-                                       frag_a8, items, n, co);
+                    hipLaunchKernelGGL((k_prefilter_mfma<false, KS>), pf_grid(nitems), dim3(256), 0, ctx->stream,
+                                       frag, frag_a8, items, nitems, pf_ctr, co);
             }
             MDET_HIP(hipGetLastError());
             if (attempt == 0) MDET_TRY(tm.record(2));
@@ -1970,7 +2006,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 rows = tnfs;
                 tv = tvs;
             }
-            uint64_t blocks = std::min<uint64_t>((ns + 63) / 64, (uint64_t)ctx->num_cus * 32);
+            uint64_t blocks = std::min<uint64_t>((ns + 63) / 64, (uint64_t)ctx->num_cus * tune.refine_blocks);
             blocks = (blocks + 7) / 8 * 8;
             hipLaunchKernelGGL(k_refine, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, surv, ns, perm, rows,
                                logsize_dev, tv, T, cand, rc, bcnt);
@@ -2089,6 +2125,136 @@ int mdet_tnf_edges_learned(mdet_ctx* ctx, mdet_mlp* mlp, const float* tnf_dev, c
                            mdet_edges* out, mdet_pair_stats* stats) {
     return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, MDET_LEARNED, mlp, radius2, n_clusters, n_probe,
                       out, stats);
+}
+
+int mdet_tune_get(mdet_ctx* ctx, int precision, mdet_tune* out) {
+    MDET_TRY(check_ctx(ctx));
+    if (precision < MDET_EXACT_F32 || precision > MDET_LEARNED || !out)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tune_get: bad precision %d or null output", precision);
+    const mdet_ctx::Tune& t = ctx->tune[precision];
+    *out = mdet_tune{t.seg_cols, t.persistent, t.refine_blocks, 0u, t.ms_prefilter, t.ms_refine};
+    return MDET_OK;
+}
+
+int mdet_tune_set(mdet_ctx* ctx, int precision, const mdet_tune* in) {
+    MDET_TRY(check_ctx(ctx));
+    if (precision < MDET_EXACT_F32 || precision > MDET_LEARNED || !in)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tune_set: bad precision %d or null input", precision);
+    if (in->seg_cols < M_ROWS || in->seg_cols > 65536 || in->seg_cols % M_ROWS)
+        MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: seg_cols %u is not a multiple of %d in [%d, 65536]", in->seg_cols,
+                  M_ROWS, M_ROWS);
+    if (in->persistent > 1) MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: persistent must be 0 or 1");
+    if (in->refine_blocks < 1 || in->refine_blocks > 64)
+        MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: refine_blocks %u outside [1, 64]", in->refine_blocks);
+    mdet_ctx::Tune& t = ctx->tune[precision];
+    t.seg_cols = in->seg_cols;
+    t.persistent = in->persistent;
+    t.refine_blocks = in->refine_blocks;
+    t.ms_prefilter = in->ms_prefilter;
+    t.ms_refine = in->ms_refine;
+    return MDET_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Times the launch shapes of the pair stage on this input and keeps the
+// fastest in ctx->tune[precision]: first the prefilter shape (item column
+// span for the Gram modes x one block per item or persistent blocks), by
+// the median prefilter stage time over reps calls, then the refine blocks
+// per CU at that shape, by the median refine stage time.  Every shape gives
+// the same edge list; the last call's list stays in ctx.
+int autotune_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
+                  uint32_t max_edges, int precision, mdet_mlp* mlp, float radius2, uint32_t n_clusters,
+                  uint32_t n_probe, uint32_t reps, mdet_tune* out) {
+    MDET_TRY(check_ctx(ctx));
+    if (precision < MDET_EXACT_F32 || precision > MDET_LEARNED)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_autotune: bad precision %d", precision);
+    reps = std::max<uint32_t>(1, std::min<uint32_t>(reps, 15));
+    mdet_ctx::Tune& t = ctx->tune[precision];
+    const mdet_ctx::Tune saved = t;
+    mdet_edges e;
+    mdet_pair_stats st;
+    auto call = [&]() {
+        return edges_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, precision, mlp, radius2, n_clusters,
+                          n_probe, &e, &st);
+    };
+    auto median_of = [&](double mdet_pair_stats::*field, double* ms) -> int {
+        std::vector<double> v;
+        for (uint32_t r = 0; r < reps; ++r) {
+            MDET_TRY(call());
+            v.push_back(st.*field);
+        }
+        std::sort(v.begin(), v.end());
+        *ms = v[v.size() / 2];
+        return MDET_OK;
+    };
+    auto fail = [&](int rc) {
+        t = saved;
+        return rc;
+    };
+    if (int rc = call(); rc != MDET_OK) return fail(rc);  // first-call allocations at the current shape
+    // 1. prefilter shape
+    std::vector<std::pair<uint32_t, uint32_t>> shapes;  // (seg_cols, persistent)
+    if (precision == MDET_BF16_REFINE || precision == MDET_F16_REFINE) {
+        for (uint32_t seg : {512u, 1024u, 2048u, 4096u})
+            for (uint32_t pers : {0u, 1u}) shapes.push_back({seg, pers});
+    } else if (precision == MDET_LEARNED) {
+        for (uint32_t pers : {0u, 1u}) shapes.push_back({t.seg_cols, pers});
+    }
+    double best_pre = 0;
+    if (!shapes.empty()) {
+        best_pre = 1e300;
+        std::pair<uint32_t, uint32_t> best = shapes[0];
+        for (const auto& sh : shapes) {
+            t.seg_cols = sh.first;
+            t.persistent = sh.second;
+            double ms;
+            if (int rc = median_of(&mdet_pair_stats::ms_prefilter, &ms); rc != MDET_OK) return fail(rc);
+            if (ms < best_pre) {
+                best_pre = ms;
+                best = sh;
+            }
+        }
+        t.seg_cols = best.first;
+        t.persistent = best.second;
+    }
+    // 2. refine blocks per CU
+    double best_ref = 1e300;
+    uint32_t best_rb = t.refine_blocks;
+    for (uint32_t rb : {8u, 16u, 32u, 64u}) {
+        t.refine_blocks = rb;
+        double ms;
+        if (int rc = median_of(&mdet_pair_stats::ms_refine, &ms); rc != MDET_OK) return fail(rc);
+        if (ms < best_ref) {
+            best_ref = ms;
+            best_rb = rb;
+        }
+    }
+    t.refine_blocks = best_rb;
+    t.ms_prefilter = best_pre;
+    t.ms_refine = best_ref;
+    if (out) *out = mdet_tune{t.seg_cols, t.persistent, t.refine_blocks, 0u, t.ms_prefilter, t.ms_refine};
+    return MDET_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdet_tnf_autotune(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
+                      uint32_t max_edges, int precision, uint32_t reps, mdet_tune* out) {
+    if (precision == MDET_LEARNED)
+        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_autotune: the learned mode needs mdet_tnf_autotune_learned");
+    return autotune_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, precision, nullptr, 0.f, 0, 0, reps, out);
+}
+
+int mdet_tnf_autotune_learned(mdet_ctx* ctx, mdet_mlp* mlp, const float* tnf_dev, const double* logsize_dev,
+                              uint32_t n, double cutoff, uint32_t max_edges, float radius2, uint32_t n_clusters,
+                              uint32_t n_probe, uint32_t reps, mdet_tune* out) {
+    return autotune_impl(ctx, tnf_dev, logsize_dev, n, cutoff, max_edges, MDET_LEARNED, mlp, radius2, n_clusters,
+                         n_probe, reps, out);
 }
 
 int mdet_last_pairs(mdet_ctx* ctx, const void** records_dev, uint64_t* count) {
