@@ -245,11 +245,12 @@ int mdet_tnf_edges_learned(mdet_ctx *ctx, mdet_mlp *mlp, const float *tnf_dev, c
  * the edge list. */
 typedef struct mdet_tune {
     uint32_t seg_cols;      /* MFMA Gram modes: prefilter item column span, a
-                             * multiple of 256 in [256, 65536] (default 1024) */
-    uint32_t persistent;    /* MFMA modes: 0 one prefilter block per item
-                             * (default), 1 resident blocks take items from a
+                             * multiple of 256 in [256, 65536], or 0 (default):
+                             * the power of two nearest n/100 in [512, 4096] */
+    uint32_t persistent;    /* MFMA modes: 0 one prefilter block per item, 1
+                             * (default) resident blocks take items from a
                              * counter */
-    uint32_t refine_blocks; /* refinement blocks per CU, 1..64 (default 32) */
+    uint32_t refine_blocks; /* refinement blocks per CU, 1..64 (default 64) */
     uint32_t reserved;      /* 0 */
     double ms_prefilter;    /* autotuner: median stage times at the chosen */
     double ms_refine;       /* shape (0 when never tuned) */

@@ -366,7 +366,7 @@ class Ctx:
         _check(lib().mdet_tune_get(self.h, precision, C.byref(t)), "mdet_tune_get")
         return t.as_dict()
 
-    def tune_set(self, precision: int, seg_cols: int = 1024, persistent: int = 0, refine_blocks: int = 32):
+    def tune_set(self, precision: int, seg_cols: int = 0, persistent: int = 1, refine_blocks: int = 64):
         t = Tune(seg_cols, persistent, refine_blocks, 0, 0.0, 0.0)
         _check(lib().mdet_tune_set(self.h, precision, C.byref(t)), "mdet_tune_set")
 

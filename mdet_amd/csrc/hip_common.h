@@ -53,9 +53,9 @@ struct mdet_ctx {
     // autotuner): prefilter item column span, persistent prefilter blocks,
     // refine blocks per CU; the autotuner's stage times at that shape
     struct Tune {
-        uint32_t seg_cols = 1024;
-        uint32_t persistent = 0;
-        uint32_t refine_blocks = 32;
+        uint32_t seg_cols = 0;  // 0: by size (seg_cols_for)
+        uint32_t persistent = 1;
+        uint32_t refine_blocks = 64;
         double ms_prefilter = 0, ms_refine = 0;
     } tune[4];
     uint64_t last_pairs = 0;  // refine records of the last edge-list call

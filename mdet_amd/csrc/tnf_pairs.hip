@@ -1687,6 +1687,15 @@ int sort_by_composition(mdet_ctx* ctx, const float* tnf_dev, uint32_t n, uint32_
 // MFMA-only blocks on every CU.  (A segment-major order with an XCD-aware block map cut
 // the prefilter's L2-miss traffic from 19 GB to 2.9 GB at 200k contigs but ran
 // 8% slower: the kernel is bound by instruction issue, not by those fetches.)
+// Default column span by size: the power of two nearest n / 100 in [512,
+// 4096] (the autotuner's picks measured at 50k, 200k and 1M contigs were
+// 512, 2048 and 4096, with persistent blocks).
+uint32_t seg_cols_for(uint32_t n) {
+    uint32_t seg = 512;
+    while (seg < 4096 && (uint64_t)seg * 3 / 2 * 100 < n) seg *= 2;
+    return seg;
+}
+
 int build_items(mdet_ctx* ctx, uint32_t n_pad, uint32_t seg, uint4** items_dev, uint32_t* nitems) {
     if (ctx->items_npad == n_pad && ctx->items_seg == seg && ctx->nitems) {
         auto it = ctx->bufs.find("pairs.items");
@@ -1895,7 +1904,8 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 }
                 uint4* items = ivf.items_dev;
                 uint32_t nitems = ivf.nitems;
-                if (!n_probe) MDET_TRY(build_items(ctx, n_pad, tune.seg_cols, &items, &nitems));
+                if (!n_probe)
+                    MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), &items, &nitems));
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (nitems)
@@ -1920,7 +1930,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                                        tnf_dev, perm, n, n_pad, R2, frag, frag_a8);
                 uint4* items;
                 uint32_t nitems;
-                MDET_TRY(build_items(ctx, n_pad, tune.seg_cols, &items, &nitems));
+                MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), &items, &nitems));
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (bf)
@@ -2140,9 +2150,9 @@ int mdet_tune_set(mdet_ctx* ctx, int precision, const mdet_tune* in) {
     MDET_TRY(check_ctx(ctx));
     if (precision < MDET_EXACT_F32 || precision > MDET_LEARNED || !in)
         MDET_FAIL(MDET_E_INVALID, "mdet_tune_set: bad precision %d or null input", precision);
-    if (in->seg_cols < M_ROWS || in->seg_cols > 65536 || in->seg_cols % M_ROWS)
-        MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: seg_cols %u is not a multiple of %d in [%d, 65536]", in->seg_cols,
-                  M_ROWS, M_ROWS);
+    if (in->seg_cols != 0 && (in->seg_cols < M_ROWS || in->seg_cols > 65536 || in->seg_cols % M_ROWS))
+        MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: seg_cols %u is neither 0 nor a multiple of %d in [%d, 65536]",
+                  in->seg_cols, M_ROWS, M_ROWS);
     if (in->persistent > 1) MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: persistent must be 0 or 1");
     if (in->refine_blocks < 1 || in->refine_blocks > 64)
         MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: refine_blocks %u outside [1, 64]", in->refine_blocks);

@@ -10,7 +10,7 @@ from mdet_amd import workload
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(seg_cols=1024, persistent=0, refine_blocks=32)
+DEFAULT = dict(seg_cols=0, persistent=1, refine_blocks=64)
 
 
 def _same(a, b):
@@ -25,8 +25,8 @@ def test_every_gram_launch_shape_gives_the_same_edges(ctx):
             base = ctx.tnf_edges(w.tnf, w.logsize, w.n, 0.95, 200, mode)
             print(f"mode {mode}: {len(base[0])} edges, {base[3].survivors} survivors", flush=True)
             assert len(base[0]) > 10000
-            shapes = [dict(seg_cols=s, persistent=p, refine_blocks=32) for s in (256, 512, 2048, 4096, 8192)
-                      for p in (0, 1)] + [dict(seg_cols=1024, persistent=1, refine_blocks=r) for r in (1, 8, 64)]
+            shapes = [dict(seg_cols=s, persistent=p, refine_blocks=32) for s in (256, 512, 1024, 4096, 8192)
+                      for p in (0, 1)] + [dict(seg_cols=0, persistent=0, refine_blocks=r) for r in (1, 8, 64)]
             for sh in shapes:
                 ctx.tune_set(mode, **sh)
                 got = ctx.tnf_edges(w.tnf, w.logsize, w.n, 0.95, 200, mode)
@@ -73,7 +73,7 @@ def test_learned_launch_shapes_give_the_same_edges(ctx):
         ctx.tune_set(M.LEARNED, **DEFAULT)
         kw = dict(mlp=mlp, radius2=r2, n_probe=3, n_clusters=9)
         base = ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, **kw)
-        for sh in (dict(seg_cols=1024, persistent=1, refine_blocks=32), dict(seg_cols=1024, persistent=0,
+        for sh in (dict(seg_cols=1024, persistent=0, refine_blocks=32), dict(seg_cols=0, persistent=1,
                                                                             refine_blocks=4)):
             ctx.tune_set(M.LEARNED, **sh)
             assert _same(ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, **kw), base), sh
@@ -89,10 +89,11 @@ def test_learned_launch_shapes_give_the_same_edges(ctx):
 
 
 def test_tune_set_rejects_bad_shapes(ctx):
+    ctx.tune_set(M.BF16_REFINE, **DEFAULT)
     for bad in (dict(seg_cols=100), dict(seg_cols=1000), dict(seg_cols=1 << 17), dict(persistent=2),
                 dict(refine_blocks=0), dict(refine_blocks=65)):
         with pytest.raises(M.MdetError, match="mdet_tune_set"):
             ctx.tune_set(M.BF16_REFINE, **{**DEFAULT, **bad})
     with pytest.raises(M.MdetError, match="precision"):
         ctx.tune_set(7, **DEFAULT)
-    assert ctx.tune_get(M.BF16_REFINE)["seg_cols"] == 1024
+    assert ctx.tune_get(M.BF16_REFINE) == {**DEFAULT, "ms_prefilter": 0.0, "ms_refine": 0.0}
