@@ -1082,7 +1082,7 @@ __device__ inline bool elem_before(float ka, uint32_t ia, float kb, uint32_t ib)
 }
 
 template <int E, bool BY_RANK>
-__device__ void wave_sort_regs(float (&key)[E], uint32_t (&id)[E], int lane) {
+__device__ __forceinline__ void wave_sort_regs(float (&key)[E], uint32_t (&id)[E], int lane) {
 #pragma unroll
     for (int k = 2; k <= 64 * E; k <<= 1)
 #pragma unroll
@@ -1142,6 +1142,29 @@ __device__ void sort_ids_and_store(const float* key, const uint32_t* id, uint32_
     }
 }
 
+// deg (<= 64 * E) entries of adj[beg..] in rank order (pre desc, partner
+// asc) into LDS key/id[0..64E), padded with (-inf, 0xffffffff) entries.
+template <int E>
+__device__ void rank_sort_to_lds(const Adj* __restrict__ adj, uint64_t beg, uint32_t deg, float* key, uint32_t* id,
+                                 int lane) {
+    float k[E];
+    uint32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t t = e * 64 + lane;
+        const bool in = t < deg;
+        const Adj a = in ? adj[beg + t] : Adj{0xffffffffu, 0.f};
+        k[e] = in ? a.pre : -__builtin_inff();
+        v[e] = in ? a.partner : 0xffffffffu;
+    }
+    wave_sort_regs<E, true>(k, v, lane);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        key[e * 64 + lane] = k[e];
+        id[e * 64 + lane] = v[e];
+    }
+}
+
 // Row `rid` (contig id), deg entries at adj[beg..): by one wave.  Returns
 // false, touching nothing in global memory, if the row needs the block path
 // (FP32 tie zone at the cut wider than a wave).
@@ -1150,17 +1173,17 @@ __device__ bool wave_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t de
                                 uint32_t* id, uint32_t* __restrict__ out_cnt, uint64_t* __restrict__ rbeg,
                                 unsigned long long* __restrict__ capped, int lane) {
     const bool cap = K && deg > K;
-    int sz;
     if (cap) {
-        sz = pow2_at_least(deg);
-        for (int t = lane; t < sz; t += 64) {
-            const bool in = t < (int)deg;
-            const Adj e = in ? adj[beg + t] : Adj{0xffffffffu, 0.f};
-            key[t] = in ? e.pre : -__builtin_inff();
-            id[t] = in ? e.partner : 0xffffffffu;
-        }
+        // rank order (pre desc, partner asc) in registers, then to LDS
+        if (deg <= 64)
+            rank_sort_to_lds<1>(adj, beg, deg, key, id, lane);
+        else if (deg <= 128)
+            rank_sort_to_lds<2>(adj, beg, deg, key, id, lane);
+        else if (deg <= 256)
+            rank_sort_to_lds<4>(adj, beg, deg, key, id, lane);
+        else
+            rank_sort_to_lds<8>(adj, beg, deg, key, id, lane);
         wave_sync();
-        wave_sort<true>(key, id, sz, lane);
         const float vk = key[K - 1];
         // the FP32 keys within 2*EPS of the K-th form a contiguous range [z0, z1)
         int z0 = (int)deg, z1 = 0;
@@ -1197,10 +1220,38 @@ __device__ bool wave_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t de
             wave_sync();
         }
         if (lane == 0) atomicAdd(capped, 1ull);
-        // keep the K selected partners above the row, then sort them by id
-        for (int t = lane; t < sz; t += 64)
-            if (t >= (int)K || id[t] <= rid) id[t] = 0xffffffffu;
+        // keep the K selected partners above the row, compacted in place
+        // (a chunk's writes land below its reads and below the next
+        // chunk's), then sort them by id
+        uint32_t cnt = 0;
+        for (uint32_t t0 = 0; t0 < K; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            const bool in = t < K;
+            const float kk = in ? key[t] : 0.f;
+            const uint32_t kv = in ? id[t] : 0xffffffffu;
+            const bool keep = in && kv > rid && kv != 0xffffffffu;
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                const uint32_t o = cnt + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                key[o] = kk;
+                id[o] = kv;
+            }
+            cnt += (uint32_t)__popcll(m);
+        }
         wave_sync();
+        if (cnt <= 64)
+            sort_ids_and_store<1>(key, id, cnt, adj, beg, lane);
+        else if (cnt <= 128)
+            sort_ids_and_store<2>(key, id, cnt, adj, beg, lane);
+        else if (cnt <= 256)
+            sort_ids_and_store<4>(key, id, cnt, adj, beg, lane);
+        else
+            sort_ids_and_store<8>(key, id, cnt, adj, beg, lane);
+        if (lane == 0) {
+            out_cnt[rid] = cnt;
+            rbeg[rid] = beg;
+        }
+        return true;
     } else {
         // every candidate is selected: gather the partners above the row,
         // then sort them by id in registers
@@ -1233,15 +1284,6 @@ __device__ bool wave_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t de
         }
         return true;
     }
-    wave_sort<false>(key, id, sz, lane);
-    uint32_t cnt = 0;
-    for (int t0 = 0; t0 < sz; t0 += 64) cnt += (uint32_t)__popcll(__ballot(t0 + lane < sz && id[t0 + lane] != 0xffffffffu));
-    for (uint32_t t = lane; t < cnt; t += 64) adj[beg + t] = Adj{id[t], key[t]};
-    if (lane == 0) {
-        out_cnt[rid] = cnt;
-        rbeg[rid] = beg;
-    }
-    return true;
 }
 
 // The same by the whole block, for rows of up to SEL_CAP entries and tie
@@ -1340,7 +1382,7 @@ __device__ void block_row_select(Adj* __restrict__ adj, uint64_t beg, uint32_t d
 // k_bigrow_select: big[0] = count, then (row id, degree, begin) triples.
 constexpr int BIG_REC = 4;  // uint32 words per big-row record: rid, deg, beg lo, beg hi
 
-__global__ __launch_bounds__(S_THREADS) void k_bucket_select(
+__global__ __launch_bounds__(S_THREADS, 4) void k_bucket_select(
     const uint64_t* __restrict__ boff, const Ent* __restrict__ ent, Adj* __restrict__ adj, uint32_t n, uint32_t K,
     const uint32_t* __restrict__ perm, const float* __restrict__ tnf, const double* __restrict__ ls,
     uint32_t* __restrict__ out_cnt, uint64_t* __restrict__ rbeg, uint32_t* __restrict__ big, uint32_t big_cap,
