@@ -11,9 +11,11 @@ run in the same process, and their edge lists must be identical.
 value = n(n-1)/2 / (seconds per step): effective pair evaluations per second.
 
 Also reported: the learned proposal filter (MLP 136-256-256-32 trained on the
-device for 20k steps on 10M pairs labelled by the exact path, embedding radius
-calibrated on the positives) with its edge recall against the exhaustive edge
-set; the counting kernel's HBM throughput; CPU baselines on the host cores
+device for 20k steps on 10M pairs labelled by the exact path; k-means clusters
+of the embeddings, each cluster paired with its 16 nearest (32 above 262,144
+contigs), the BF16 TNF Gram and FP32 refinement within those cluster pairs)
+with its edge recall against the exhaustive edge set; the launch shapes the
+autotuner picks; the counting kernel's HBM throughput; CPU baselines on the host cores
 (the reference's own openmp/TNF.cpp for counting when it is built next to the
 oracle, the oracle's C restatement of gen_tnf_graph for pairs).
 """
@@ -135,10 +137,12 @@ def cpu_count_baseline(w, cfg_name, budget_contigs=30000):
                       f"openmp/TNF.cpp counts), its own timer around the TNF loop: {secs:.3f} s"}
 
 
-def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, autotune=True):
+def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, autotune=True, use_radius=False):
     """Train the filter on pairs the exact path labels, then time the learned
-    edge list (cluster index over the embeddings, n_probe nearest clusters)
-    and measure its recall against the exhaustive edge set."""
+    edge list (cluster index over the embeddings, n_probe nearest clusters;
+    within the probed cluster pairs the BF16 TNF Gram decides, or with
+    use_radius the embedding radius) and measure its recall against the
+    exhaustive edge set."""
     n = w.n
     t0 = time.perf_counter()
     pi, pj, y, pos = L.training_set(ctx, w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, n_pairs=10_000_000, seed=1)
@@ -149,7 +153,7 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, au
     curve, train_s = L.train(ctx, mlp, w.tnf, pi, pj, y, train_steps, batch=512, lr=2e-3, margin=4.0,
                              log_every=max(1, train_steps // 10))
     emb = L.embeddings(ctx, mlp, w.tnf, n)
-    r2 = L.calibrate_radius2(emb, pos["i"], pos["j"], quantile=quantile)
+    r2 = L.calibrate_radius2(emb, pos["i"], pos["j"], quantile=quantile) if use_radius else 0.0
     kw = dict(mlp=mlp, radius2=r2, n_probe=n_probe)
     tune = ctx.autotune(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, reps=2, **kw) if autotune \
         else ctx.tune_get(M.LEARNED)
@@ -168,6 +172,26 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, au
     got = f.astype(np.uint64) * n + t.astype(np.uint64)
     hit = np.isin(exact, got)
     mlp.close()
+    # the same filter with an untrained network (what the training buys)
+    raw = M.Mlp(ctx, seed=7)
+    raw.set_input_norm(mean, istd)
+    kw_raw = dict(kw, mlp=raw)
+    if use_radius:
+        kw_raw["radius2"] = L.calibrate_radius2(L.embeddings(ctx, raw, w.tnf, n), pos["i"], pos["j"],
+                                                quantile=quantile)
+    ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, **kw_raw)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, fetch=False, **kw_raw)
+    ctx.sync()
+    raw_sec = (time.perf_counter() - t0) / 3
+    fr, tr, _, str_ = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, **kw_raw)
+    raw_hit = np.isin(exact, fr.astype(np.uint64) * n + tr.astype(np.uint64))
+    raw.close()
+    untrained = {"ms_per_step": raw_sec * 1e3, "candidate_pairs": str_.pairs_evaluated,
+                 "survivors_exact_evaluated": str_.survivors,
+                 "edge_recall": float(raw_hit.mean()) if len(exact) else 1.0}
     return {"pairs_per_s": n * (n - 1) // 2 / sec, "ms_per_step": sec * 1e3,
             "stage_ms": {k: float(np.mean([x[k] for x in stats]))
                          for k in ("ms_prefilter", "ms_refine", "ms_select", "ms_total")},
@@ -175,7 +199,9 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, au
             "survivors_exact_evaluated": stats[-1]["survivors"], "edges": int(len(f)),
             "edge_recall": float(hit.mean()) if len(exact) else 1.0,
             "edges_not_in_exhaustive_set": int((~np.isin(got, exact)).sum()),
-            "radius2": r2, "radius_quantile": quantile, "tune": tune,
+            "within_cluster_pairs": "embedding radius" if use_radius else "BF16 TNF Gram prefilter",
+            "untrained_network_same_probes": untrained,
+            "radius2": r2, "radius_quantile": quantile if use_radius else None, "tune": tune,
             "training": {"pairs": int(len(pi)), "positives": int(y.sum()), "steps": train_steps, "batch": 512,
                          "seconds": train_s, "ms_per_step": 1e3 * train_s / max(1, train_steps),
                          "set_build_s": set_s, "loss_curve": [(int(a), float(b)) for a, b in curve]}}
@@ -239,6 +265,9 @@ def main():
     ap.add_argument("--modes", default="exact_f32,f16_refine")
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-radius", action="store_true",
+                    help="learned filter: embedding radius test within the probed cluster pairs "
+                         "(default: the BF16 TNF Gram decides there)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the BF16 and learned paths at the default launch shapes")
     ap.add_argument("--n-probe", type=int, default=0,
@@ -304,7 +333,8 @@ def main():
     if not args.no_learned:
         log(f"learned filter: {args.train_steps} training steps")
         n_probe = args.n_probe or (16 if w.n <= 262144 else 32)
-        lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, n_probe, autotune=not args.no_autotune)
+        lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, n_probe, autotune=not args.no_autotune,
+                         use_radius=args.learned_radius)
         lm["speedup_vs_bf16_refine"] = lm["pairs_per_s"] / modes["bf16_refine"]["pairs_per_s"]
         modes["learned"] = lm
 

@@ -229,8 +229,11 @@ int mdet_mlp_embed(mdet_mlp *mlp, const float *tnf_dev, uint32_t n, float *emb_d
  * Gram of the 32-dimensional embeddings over the whole triangle); with
  * n_probe > 0 (at most 32) a cluster index over the embeddings (k-means,
  * n_clusters centroids, 0 = n/192, at most 8192) and only the pairs of
- * clusters among each other's n_probe nearest centroids.  Edges of pairs the filter fails to
- * propose are missed: recall is measured against mdet_tnf_edges.
+ * clusters among each other's n_probe nearest centroids.  With n_probe > 0
+ * and radius2 == 0 the embeddings only choose the cluster pairs: the BF16
+ * TNF Gram prefilter of MDET_BF16_REFINE decides within them (fewer pairs
+ * for the exact stage, no radius losses).  Edges of pairs the filter fails
+ * to propose are missed: recall is measured against mdet_tnf_edges.
  * stats->pairs_evaluated counts the candidate pairs. */
 int mdet_tnf_edges_learned(mdet_ctx *ctx, mdet_mlp *mlp, const float *tnf_dev, const double *logsize_dev, uint32_t n,
                            double cutoff, uint32_t max_edges, float radius2, uint32_t n_clusters, uint32_t n_probe,

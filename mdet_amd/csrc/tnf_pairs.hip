@@ -1783,8 +1783,9 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
     if (!(cutoff > 0.0 && cutoff < 1.0)) MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: cutoff %g not in (0, 1)", cutoff);
     if (precision < MDET_EXACT_F32 || precision > MDET_LEARNED)
         MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges: unknown precision %d", precision);
-    if (precision == MDET_LEARNED && (!mlp || !(radius2 > 0)))
-        MDET_FAIL(MDET_E_INVALID, "mdet_tnf_edges_learned: needs a network and a positive radius");
+    if (precision == MDET_LEARNED && (!mlp || !(radius2 > 0 || (radius2 == 0.f && n_probe > 0))))
+        MDET_FAIL(MDET_E_INVALID,
+                  "mdet_tnf_edges_learned: needs a network and a positive radius (radius2 = 0 only with probes)");
     if (n > (1u << 30)) MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: n = %u too large", n);
     if (n < 2) {
         if (stats) *stats = st;
@@ -1903,6 +1904,14 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             rows_sorted = true;
             return MDET_OK;
         };
+        // squared-distance bound of the BF16/FP16 Gram prefilter: |d~ - d| <=
+        // |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit roundoff of the
+        // format), plus FP32 accumulation slack
+        auto gram_r2 = [&](bool bf) {
+            const double u = bf ? 1.0 / 256.0 : 1.0 / 2048.0;
+            const double Rm = R + 2.0 * u * 1.0001 + 2e-4;
+            return (float)(Rm * Rm) + 1e-4f;
+        };
         for (int attempt = 0; attempt < 2; ++attempt) {
             MDET_HIP(hipMemsetAsync(scnt, 0, (size_t)NSHARD * SHARD_STRIDE * 8, ctx->stream));
             MDET_HIP(hipMemsetAsync(sstat, 0, 8 * 8, ctx->stream));
@@ -1922,7 +1931,11 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             } else if (precision == MDET_LEARNED) {
                 // embedding rows (n x 32) in BF16, threshold = calibrated radius^2;
                 // with probes, only the pairs of probed cluster pairs (items of
-                // the cluster index), else every pair in composition order
+                // the cluster index), else every pair in composition order.
+                // radius2 == 0 with probes: the BF16 TNF Gram (as BF16_REFINE)
+                // decides within the probed cluster pairs instead of the
+                // embedding radius.
+                const bool tnf_gram = n_probe && radius2 == 0.f;
                 float* emb;
                 MDET_TRY(ctx_buf(ctx, "learn.emb", (size_t)n * 32, &emb));
                 if (attempt == 0) {
@@ -1932,14 +1945,18 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 // cluster items start column steps anywhere: room for one more block
                 const uint32_t n_frag = n_probe ? n_pad + ITEM_ROWS : n_pad;
                 uint4 *frag, *frag_a;
-                MDET_TRY(ctx_buf(ctx, "learn.frag", (size_t)n_frag / 32 * KS_E * 64, &frag));
+                MDET_TRY(ctx_buf(ctx, "learn.frag", (size_t)n_frag / 32 * (tnf_gram ? KS : KS_E) * 64, &frag));
                 MDET_TRY(ctx_buf(ctx, "learn.frag_a", (size_t)n_frag / 32 * 64, &frag_a));
                 if (n_probe)
                     perm = ivf.perm;
                 else
                     MDET_TRY(sort_by_composition(ctx, tnf_dev, n, &perm));
-                hipLaunchKernelGGL((k_prep_lowp<true, 32, KS_E>), dim3(n_frag / 4), dim3(256), 0, ctx->stream, emb,
-                                   perm, n, n_frag, radius2, frag, frag_a);
+                if (tnf_gram)
+                    hipLaunchKernelGGL((k_prep_lowp<true, NT, KS>), dim3(n_frag / 4), dim3(256), 0, ctx->stream,
+                                       tnf_dev, perm, n, n_frag, gram_r2(true), frag, frag_a);
+                else
+                    hipLaunchKernelGGL((k_prep_lowp<true, 32, KS_E>), dim3(n_frag / 4), dim3(256), 0, ctx->stream,
+                                       emb, perm, n, n_frag, radius2, frag, frag_a);
                 if (n_probe && attempt == 0) {
                     MDET_TRY(sort_rows());         // device work queued behind the index copies ...
                     MDET_TRY(ivf_finish(ctx, &ivf));  // ... runs while the host builds the items
@@ -1950,16 +1967,15 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                     MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), &items, &nitems));
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
-                if (nitems)
+                if (nitems && tnf_gram)
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), pf_grid(nitems), dim3(256), 0, ctx->stream,
+                                       frag, frag_a, items, nitems, pf_ctr, co);
+                else if (nitems)
                     hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), pf_grid(nitems), dim3(256), 0, ctx->stream,
                                        frag, frag_a, items, nitems, pf_ctr, co);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
-                // |d~ - d| <= |a~ - a| + |b~ - b| <= 2u for unit rows (u = unit
-                // roundoff of the format), plus FP32 accumulation slack
-                const double u = bf ? 1.0 / 256.0 : 1.0 / 2048.0;
-                const double Rm = R + 2.0 * u * 1.0001 + 2e-4;
-                const float R2 = (float)(Rm * Rm) + 1e-4f;
+                const float R2 = gram_r2(bf);
                 uint4 *frag, *frag_a8;
                 MDET_TRY(ctx_buf(ctx, "pairs.frag", (size_t)n_pad / 32 * KS * 64, &frag));
                 MDET_TRY(ctx_buf(ctx, "pairs.frag_a8", (size_t)n_pad / 32 * 64, &frag_a8));

@@ -174,6 +174,32 @@ def test_cluster_index_with_every_cluster_probed_equals_exhaustive(ctx):
     mlp.close()
 
 
+def test_cluster_index_with_tnf_gram_and_every_cluster_probed_equals_bf16(ctx):
+    """radius2 = 0: the BF16 TNF Gram decides within the probed cluster pairs,
+    so probing every cluster gives exactly the exhaustive BF16 edge list."""
+    w = workload.make(ctx, "c1_2k", seed=5)
+    n = w.n
+    mean, istd = L.input_norm(w.tnf.numpy())
+    mlp = M.Mlp(ctx, seed=3)
+    mlp.set_input_norm(mean, istd)
+    f0, t0, s0, st0 = ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.BF16_REFINE)
+    k = 9
+    f1, t1, s1, st1 = ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, mlp=mlp, radius2=0.0, n_probe=k,
+                                    n_clusters=k)
+    print(f"BF16: {st0.survivors} survivors, {len(f0)} edges; TNF Gram over all {k} clusters: "
+          f"{st1.pairs_evaluated} pairs, {st1.survivors} survivors", flush=True)
+    assert st1.pairs_evaluated == st0.pairs_total
+    assert st1.survivors == st0.survivors
+    assert np.array_equal(f0, f1) and np.array_equal(t0, t1) and np.array_equal(s0, s1)
+    f2, t2, s2, st2 = ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, mlp=mlp, radius2=0.0, n_probe=2,
+                                    n_clusters=k)
+    assert st2.pairs_evaluated < st0.pairs_total and st2.survivors <= st0.survivors
+    assert len(f2) > 0 and np.all(f2 < t2)
+    with pytest.raises(M.MdetError, match="radius"):
+        ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, mlp=mlp, radius2=0.0, n_probe=0)
+    mlp.close()
+
+
 def test_cluster_index_recall_after_training(ctx):
     w = workload.make(ctx, "c1_2k", seed=11)
     n = w.n
