@@ -760,6 +760,9 @@ __device__ __forceinline__ void prefilter_item(const uint4* __restrict__ frag, c
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = f32x16_t{};
+            // let the scheduler interleave the B-fragment LDS reads with the
+            // MFMA chains (measured 2% faster at 200k contigs)
+            __builtin_amdgcn_iglp_opt(0);
 #pragma unroll
             for (int ks = 0; ks < KSX; ++ks) {
                 uint4 b[2];
