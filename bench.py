@@ -12,7 +12,7 @@ value = n(n-1)/2 / (seconds per step): effective pair evaluations per second.
 
 Also reported: the learned proposal filter (MLP 136-256-256-32 trained on the
 device for 20k steps on 10M pairs labelled by the exact path; k-means clusters
-of the embeddings, each cluster paired with its 16 nearest (32 above 262,144
+of the embeddings, each cluster paired with its 20 nearest (32 above 262,144
 contigs), the BF16 TNF Gram and FP32 refinement within those cluster pairs)
 with its edge recall against the exhaustive edge set; the launch shapes the
 autotuner picks; the counting kernel's HBM throughput; CPU baselines on the host cores
@@ -271,7 +271,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the BF16 and learned paths at the default launch shapes")
     ap.add_argument("--n-probe", type=int, default=0,
-                    help="clusters probed per cluster by the learned filter (0: 16 up to 262,144 contigs, else 32)")
+                    help="clusters probed per cluster by the learned filter (0: 20 up to 262,144 contigs, else 32)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain copy of this size (counter calibration) and exit")
     args = ap.parse_args()
@@ -332,7 +332,7 @@ def main():
         modes["bf16_refine"]["speedup_vs_exact_f32"] = modes["bf16_refine"]["pairs_per_s"] / modes["exact_f32"]["pairs_per_s"]
     if not args.no_learned:
         log(f"learned filter: {args.train_steps} training steps")
-        n_probe = args.n_probe or (16 if w.n <= 262144 else 32)
+        n_probe = args.n_probe or (20 if w.n <= 262144 else 32)
         lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, n_probe, autotune=not args.no_autotune,
                          use_radius=args.learned_radius)
         lm["speedup_vs_bf16_refine"] = lm["pairs_per_s"] / modes["bf16_refine"]["pairs_per_s"]

@@ -37,9 +37,24 @@ __device__ inline float bf16_round(float f) {
     return __uint_as_float(u & 0xffff0000u);
 }
 
-__device__ inline float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, formula
+// 7.1.26: for u >= 0, erf(u) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5)
+// exp(-u^2) with t = 1 / (1 + p u), |error| <= 1.5e-7; odd in u.  Branch-free
+// (one reciprocal, one exponential), far inside the BF16 rounding of the
+// activations it feeds.
+__device__ inline float erf_as(float u) {
+    const float a = fabsf(u);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.f));
+    float poly = fmaf(1.061405429f, t, -1.453152027f);
+    poly = fmaf(poly, t, 1.421413741f);
+    poly = fmaf(poly, t, -0.284496736f);
+    poly = fmaf(poly, t, 0.254829592f);
+    const float r = 1.f - poly * t * __expf(-a * a);
+    return copysignf(r, u);
+}
+__device__ inline float gelu(float x) { return 0.5f * x * (1.f + erf_as(x * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float x) {
-    const float cdf = 0.5f * (1.f + erff(x * 0.70710678118654752f));
+    const float cdf = 0.5f * (1.f + erf_as(x * 0.70710678118654752f));
     const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
     return cdf + x * pdf;
 }
