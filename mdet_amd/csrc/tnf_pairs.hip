@@ -604,8 +604,9 @@ __global__ __launch_bounds__(256) void k_prefilter_f32(const float* __restrict__
 // the autotuner): at 200k contigs 1024 columns ran the step 3% faster than
 // 2048 (finer scheduling of the survivor-heavy diagonal items) with shorter
 // survivor runs for refinement; 512 and 768 were no faster overall.
-constexpr int M_ROWS = 256, M_STEP = 128;
-constexpr int M_MIN_BLOCKS = 2;  // resident blocks per CU the registers are sized for
+constexpr int M_GRAM_WAVES = 4;  // waves (64 rows each) per prefilter block of the Gram modes
+constexpr int M_ROWS = 64 * M_GRAM_WAVES, M_STEP = 128;
+// the registers are sized for 2 waves per SIMD: 8 resident waves per CU
 // Prefilter item: (r_lo, r_hi, c_lo, c_hi | ITEM_TRI) = pairs of row positions
 // [r_lo, r_hi) and column positions [c_lo, c_hi), only i < j if ITEM_TRI;
 // r_hi - (r_lo & ~31) <= M_ROWS.  The fragment arrays must hold whole 128-
@@ -623,16 +624,16 @@ __device__ inline f32x16_t mfma32(uint4 a, uint4 b, f32x16_t c) {
 
 // One step's column fragments (M_STEP / 32 tiles x KSX k-steps x 64 lanes of
 // 16 bytes) into LDS: each wave copies 1 KiB chunks, chunk m*4 + wave.
-template <int KSX>
+template <int KSX, int THREADS>
 __device__ inline void issue_btile(const uint4* __restrict__ frag, uint32_t c0, uint4* lds, int tid, int wave) {
     constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;
     const uint4* src = frag + (size_t)(c0 / 32) * KSX * 64;
 #pragma unroll
-    for (int m = 0; m < (STEP_U4 + 255) / 256; ++m)
-        if (STEP_U4 % 256 == 0 || 256 * m + 64 * wave < STEP_U4)
-            __builtin_amdgcn_global_load_lds((const void*)(src + 256 * m + tid),
-                                             (__attribute__((address_space(3))) void*)(lds + 256 * m + 64 * wave), 16,
-                                             0, 0);
+    for (int m = 0; m < (STEP_U4 + THREADS - 1) / THREADS; ++m)
+        if (STEP_U4 % THREADS == 0 || THREADS * m + 64 * wave < STEP_U4)
+            __builtin_amdgcn_global_load_lds((const void*)(src + THREADS * m + tid),
+                                             (__attribute__((address_space(3))) void*)(lds + THREADS * m + 64 * wave),
+                                             16, 0, 0);
 }
 
 // Per-wave survivor staging in LDS: survivors of a step are appended to a
@@ -710,7 +711,7 @@ __global__ __launch_bounds__(256) void k_chunk_compact(const uint2* __restrict__
     for (uint32_t e = lane; e < c; e += 64) dense[d + e] = src[e];
 }
 
-template <bool BF16, int KSX>
+template <bool BF16, int KSX, int WAVES>
 __device__ __forceinline__ void prefilter_item(const uint4* __restrict__ frag, const uint4* __restrict__ frag_a8,
                                                const uint4 it, const uint32_t item,
                                                uint4 (*Bl)[M_STEP / 32 * KSX * 64], uint2 (*stage_all)[M_STAGE],
@@ -727,7 +728,7 @@ __device__ __forceinline__ void prefilter_item(const uint4* __restrict__ frag, c
     uint2* stage = stage_all[wave];
     uint32_t nstage = 0;
 
-    issue_btile<KSX>(frag, c_begin, Bl[0], tid, wave);
+    issue_btile<KSX, 64 * WAVES>(frag, c_begin, Bl[0], tid, wave);
     uint4 a[2][KSX];
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) {
@@ -745,7 +746,7 @@ __device__ __forceinline__ void prefilter_item(const uint4* __restrict__ frag, c
     for (uint32_t c0 = c_begin; c0 < c_hi; c0 += M_STEP, buf ^= 1) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (c0 + M_STEP < c_hi) issue_btile<KSX>(frag, c0 + M_STEP, Bl[buf ^ 1], tid, wave);
+        if (c0 + M_STEP < c_hi) issue_btile<KSX, 64 * WAVES>(frag, c0 + M_STEP, Bl[buf ^ 1], tid, wave);
         if (!active) continue;
         const uint4* B = Bl[buf];
         // pairs with j <= i exist only where the step meets the diagonal
@@ -820,14 +821,14 @@ __device__ __forceinline__ void prefilter_item(const uint4* __restrict__ frag, c
 
 // One block per item (item_ctr == nullptr: item = blockIdx.x), or persistent
 // blocks that take items from *item_ctr until nitems are handed out.
-template <bool BF16, int KSX>
-__global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint4* __restrict__ frag,
+template <bool BF16, int KSX, int WAVES = 4>
+__global__ __launch_bounds__(64 * WAVES, 2) void k_prefilter_mfma(const uint4* __restrict__ frag,
                                                            const uint4* __restrict__ frag_a8,
                                                            const uint4* __restrict__ items, uint32_t nitems,
                                                            uint32_t* __restrict__ item_ctr, ChunkOut out) {
     constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;  // uint4 per B step
     __shared__ uint4 Bl[2][STEP_U4];
-    __shared__ uint2 stage_all[4][M_STAGE];
+    __shared__ uint2 stage_all[WAVES][M_STAGE];
     __shared__ uint32_t s_item;
     uint32_t item = blockIdx.x;
     if (item_ctr) {
@@ -836,7 +837,7 @@ __global__ __launch_bounds__(256, M_MIN_BLOCKS) void k_prefilter_mfma(const uint
         item = s_item;
     }
     while (item < nitems) {
-        prefilter_item<BF16, KSX>(frag, frag_a8, items[item], item, Bl, stage_all, out);
+        prefilter_item<BF16, KSX, WAVES>(frag, frag_a8, items[item], item, Bl, stage_all, out);
         if (!item_ctr) break;
         __syncthreads();  // every wave is done with Bl and s_item
         if (threadIdx.x == 0) s_item = atomicAdd(item_ctr, 1u);
@@ -1019,32 +1020,8 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Bitonic sorts of sz (power of two) LDS entries by one wave (BY_RANK: by
-// ranks_before; else by id ascending), the key carried along.
-template <bool BY_RANK>
-__device__ void wave_sort(float* key, uint32_t* id, int sz, int lane) {
-    for (int k = 2; k <= sz; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = lane; t < sz; t += 64) {
-                const int u = t ^ j;
-                if (u > t) {
-                    const bool up = (t & k) == 0;
-                    const bool before = BY_RANK ? ranks_before(key[u], id[u], key[t], id[t]) : (id[u] < id[t]);
-                    if (before == up) {
-                        const float tk = key[t];
-                        key[t] = key[u];
-                        key[u] = tk;
-                        const uint32_t ti = id[t];
-                        id[t] = id[u];
-                        id[u] = ti;
-                    }
-                }
-            }
-            wave_sync();
-        }
-}
-
-// The same by a whole block (blockDim.x threads).
+// Bitonic sort of sz (power of two) LDS entries by the whole block (BY_RANK:
+// by ranks_before; else by id ascending), the key carried along.
 template <bool BY_RANK>
 __device__ void block_sort(float* key, uint32_t* id, int sz) {
     for (int k = 2; k <= sz; k <<= 1)
@@ -1872,10 +1849,10 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         // a wave flushes once per item it has survivors in, plus once per
         // M_STAGE survivors; sized for ~128 survivors per contig at first
         const mdet_ctx::Tune tune = ctx->tune[precision];
-        // persistent prefilter: M_MIN_BLOCKS resident blocks per CU pull items from a counter
+        // persistent prefilter: the resident blocks (8 waves per CU) pull items from a counter
         uint32_t* pf_ctr = nullptr;
-        auto pf_grid = [&](uint32_t nitems) {
-            return dim3(pf_ctr ? std::min<uint32_t>(nitems, (uint32_t)ctx->num_cus * M_MIN_BLOCKS) : nitems);
+        auto pf_grid = [&](uint32_t nitems, int waves) {
+            return dim3(pf_ctr ? std::min<uint32_t>(nitems, (uint32_t)ctx->num_cus * (8 / waves)) : nitems);
         };
         auto chunk_out = [&](uint32_t nitems, int attempt) -> int {
             if (tune.persistent) {
@@ -1971,11 +1948,15 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (nitems && tnf_gram)
-                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), pf_grid(nitems), dim3(256), 0, ctx->stream,
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), pf_grid(nitems, 4), dim3(256), 0, ctx->stream,
                                        frag, frag_a, items, nitems, pf_ctr, co);
-                else if (nitems)
-                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), pf_grid(nitems), dim3(256), 0, ctx->stream,
-                                       frag, frag_a, items, nitems, pf_ctr, co);
+                else if (nitems && n_probe)  // cluster items: at most ITEM_ROWS rows
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), pf_grid(nitems, 4), dim3(256), 0,
+                                       ctx->stream, frag, frag_a, items, nitems, pf_ctr, co);
+                else if (nitems)  // row-block items of M_ROWS rows
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E, M_GRAM_WAVES>), pf_grid(nitems, M_GRAM_WAVES),
+                                       dim3(64 * M_GRAM_WAVES), 0, ctx->stream, frag, frag_a, items, nitems, pf_ctr,
+                                       co);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
                 const float R2 = gram_r2(bf);
@@ -1995,11 +1976,13 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (bf)
-                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), pf_grid(nitems), dim3(256), 0, ctx->stream, frag,
-                                       frag_a8, items, nitems, pf_ctr, co);
+                    hipLaunchKernelGGL((k_prefilter_mfma<true, KS, M_GRAM_WAVES>), pf_grid(nitems, M_GRAM_WAVES),
+                                       dim3(64 * M_GRAM_WAVES), 0, ctx->stream, frag, frag_a8, items, nitems, pf_ctr,
+                                       co);
                 else
-                    hipLaunchKernelGGL((k_prefilter_mfma<false, KS>), pf_grid(nitems), dim3(256), 0, ctx->stream,
-                                       frag, frag_a8, items, nitems, pf_ctr, co);
+                    hipLaunchKernelGGL((k_prefilter_mfma<false, KS, M_GRAM_WAVES>), pf_grid(nitems, M_GRAM_WAVES),
+                                       dim3(64 * M_GRAM_WAVES), 0, ctx->stream, frag, frag_a8, items, nitems, pf_ctr,
+                                       co);
             }
             MDET_HIP(hipGetLastError());
             if (attempt == 0) MDET_TRY(tm.record(2));
