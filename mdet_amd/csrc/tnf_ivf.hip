@@ -382,11 +382,17 @@ __global__ __launch_bounds__(256) void k_km_probe(const float* __restrict__ cent
         if (counts[b] == 0) continue;
         float dist = -1.f;
         if (b != a) {
+            const float4* cb = reinterpret_cast<const float4*>(cent + (size_t)b * E_D);
             float s = 0.f;
 #pragma unroll
-            for (int d = 0; d < E_D; ++d) {
-                const float t = ca[d] - cent[(size_t)b * E_D + d];
-                s = fmaf(t, t, s);
+            for (int v = 0; v < E_D / 4; ++v) {
+                const float4 y = cb[v];
+                const float t0 = ca[4 * v] - y.x, t1 = ca[4 * v + 1] - y.y, t2 = ca[4 * v + 2] - y.z,
+                            t3 = ca[4 * v + 3] - y.w;
+                s = fmaf(t0, t0, s);
+                s = fmaf(t1, t1, s);
+                s = fmaf(t2, t2, s);
+                s = fmaf(t3, t3, s);
             }
             dist = s;
         }
@@ -483,11 +489,13 @@ int ivf_launch(mdet_ctx* ctx, const float* emb, uint32_t n, uint32_t k, uint32_t
     MDET_HIP(hipMemsetAsync(err, 0, 4, s));
     const uint32_t stride = std::max<uint32_t>(1, n / std::max<uint32_t>(1, k * km_sample_per_cluster(k)));
     const uint32_t m = (n - 1) / stride + 1;
-    // 4 row tiles x 4 waves per block where that still gives >= 4 blocks per
-    // CU, else 1 row tile x 8 waves (the centroids split over more waves)
+    // 2 row tiles x 4 waves per block where that still gives >= 8 blocks per
+    // CU (measured 0.7 ms faster at 1M contigs than 4 row tiles, which hold
+    // 2 waves per SIMD instead of 4), else 1 row tile x 8 waves (the
+    // centroids split over more waves)
     auto assign_pass = [&](uint32_t pts, uint32_t str, uint32_t* out, unsigned long long* sm, int mode) {
-        if ((pts + 127) / 128 >= 4u * (uint32_t)ctx->num_cus)
-            hipLaunchKernelGGL((k_km_assign<4, 4>), dim3((pts + 127) / 128), dim3(256), 0, s, emb, pts, str, frame,
+        if ((pts + 63) / 64 >= 8u * (uint32_t)ctx->num_cus)
+            hipLaunchKernelGGL((k_km_assign<2, 4>), dim3((pts + 63) / 64), dim3(256), 0, s, emb, pts, str, frame,
                                cfrag, k, out, counts, sm, mode, err);
         else
             hipLaunchKernelGGL((k_km_assign<1, 8>), dim3((pts + 31) / 32), dim3(512), 0, s, emb, pts, str, frame,
