@@ -254,7 +254,9 @@ typedef struct mdet_tune {
                              * (default) resident blocks take items from a
                              * counter */
     uint32_t refine_blocks; /* refinement blocks per CU, 1..64 (default 64) */
-    uint32_t reserved;      /* 0 */
+    uint32_t item_order;    /* Gram modes with persistent blocks: 0 one item
+                             * queue, row-block major; 1 (default) one queue
+                             * per XCD (row blocks rb % 8), column-start major */
     double ms_prefilter;    /* autotuner: median stage times at the chosen */
     double ms_refine;       /* shape (0 when never tuned) */
 } mdet_tune;
@@ -262,11 +264,11 @@ int mdet_tune_get(mdet_ctx *ctx, int precision, mdet_tune *out);
 int mdet_tune_set(mdet_ctx *ctx, int precision, const mdet_tune *in);
 /* Times the candidate shapes on this input (each the median of reps calls of
  * mdet_tnf_edges; reps 1..15): prefilter column spans 512/1024/2048/4096 x
- * per-item or persistent blocks (for the Gram modes; the exact mode has no
- * prefilter shape) by the prefilter stage time, then refine blocks per CU
- * 8/16/32/64 by the refine stage time.  Keeps the fastest in ctx and returns
+ * {per-item blocks, persistent blocks with item order 0 or 1} (for the Gram
+ * modes; the exact mode has no prefilter shape) by the prefilter stage time,
+ * then refine blocks per CU 8/16/32/64 by the refine stage time.  Keeps the fastest in ctx and returns
  * it (out may be NULL); the last call's edge list stays in ctx as after
- * mdet_tnf_edges.  ~(8 + 4) x reps + 1 edge-list calls. */
+ * mdet_tnf_edges.  ~(12 + 4) x reps + 1 edge-list calls. */
 int mdet_tnf_autotune(mdet_ctx *ctx, const float *tnf_dev, const double *logsize_dev, uint32_t n, double cutoff,
                       uint32_t max_edges, int precision, uint32_t reps, mdet_tune *out);
 /* The same for the learned mode (per-item or persistent prefilter blocks,

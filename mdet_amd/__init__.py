@@ -46,10 +46,10 @@ class PairStats(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("seg_cols", C.c_uint32), ("persistent", C.c_uint32), ("refine_blocks", C.c_uint32),
-                ("reserved", C.c_uint32), ("ms_prefilter", C.c_double), ("ms_refine", C.c_double)]
+                ("item_order", C.c_uint32), ("ms_prefilter", C.c_double), ("ms_refine", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -366,8 +366,9 @@ class Ctx:
         _check(lib().mdet_tune_get(self.h, precision, C.byref(t)), "mdet_tune_get")
         return t.as_dict()
 
-    def tune_set(self, precision: int, seg_cols: int = 0, persistent: int = 1, refine_blocks: int = 64):
-        t = Tune(seg_cols, persistent, refine_blocks, 0, 0.0, 0.0)
+    def tune_set(self, precision: int, seg_cols: int = 0, persistent: int = 1, refine_blocks: int = 64,
+                 item_order: int = 1):
+        t = Tune(seg_cols, persistent, refine_blocks, item_order, 0.0, 0.0)
         _check(lib().mdet_tune_set(self.h, precision, C.byref(t)), "mdet_tune_set")
 
     def autotune(self, tnf: DevArray, logsize: DevArray, n: int, cutoff: float, max_edges: int, precision: int,

@@ -48,14 +48,17 @@ struct mdet_ctx {
     // cached MFMA work items for a padded row count and column span
     uint32_t items_npad = 0;
     uint32_t items_seg = 0;
+    uint32_t items_order = 0;
     uint32_t nitems = 0;
     // pair-stage launch shapes per precision mode (set by hand or by the
     // autotuner): prefilter item column span, persistent prefilter blocks,
-    // refine blocks per CU; the autotuner's stage times at that shape
+    // refine blocks per CU, item order of the persistent blocks; the
+    // autotuner's stage times at that shape
     struct Tune {
         uint32_t seg_cols = 0;  // 0: by size (seg_cols_for)
         uint32_t persistent = 1;
         uint32_t refine_blocks = 64;
+        uint32_t item_order = 1;  // 1: per-XCD queues, column-segment major
         double ms_prefilter = 0, ms_refine = 0;
     } tune[4];
     uint64_t last_pairs = 0;  // refine records of the last edge-list call

@@ -820,27 +820,41 @@ __device__ __forceinline__ void prefilter_item(const uint4* __restrict__ frag, c
 }
 
 // One block per item (item_ctr == nullptr: item = blockIdx.x), or persistent
-// blocks that take items from *item_ctr until nitems are handed out.
+// blocks that take items from a counter until the items are handed out.
+// With qbounds (item_order 1), the items come in 8 queues [qbounds[q],
+// qbounds[q + 1]), one per XCD: block b serves queue b % 8 (the XCD
+// workgroups are dispatched to round-robin) from counter item_ctr[16 q].
+constexpr int PF_QUEUES = 8;
+constexpr int PF_CTR_STRIDE = 16;  // counters 64 bytes apart
+
 template <bool BF16, int KSX, int WAVES = 4>
 __global__ __launch_bounds__(64 * WAVES, 2) void k_prefilter_mfma(const uint4* __restrict__ frag,
                                                            const uint4* __restrict__ frag_a8,
                                                            const uint4* __restrict__ items, uint32_t nitems,
-                                                           uint32_t* __restrict__ item_ctr, ChunkOut out) {
+                                                           uint32_t* __restrict__ item_ctr,
+                                                           const uint32_t* __restrict__ qbounds, ChunkOut out) {
     constexpr int STEP_U4 = M_STEP / 32 * KSX * 64;  // uint4 per B step
     __shared__ uint4 Bl[2][STEP_U4];
     __shared__ uint2 stage_all[WAVES][M_STAGE];
     __shared__ uint32_t s_item;
-    uint32_t item = blockIdx.x;
+    uint32_t item = blockIdx.x, q_end = nitems, q_base = 0;
+    uint32_t* ctr = item_ctr;
     if (item_ctr) {
-        if (threadIdx.x == 0) s_item = atomicAdd(item_ctr, 1u);
+        if (qbounds) {
+            const uint32_t q = blockIdx.x % PF_QUEUES;
+            q_base = qbounds[q];
+            q_end = qbounds[q + 1];
+            ctr = item_ctr + q * PF_CTR_STRIDE;
+        }
+        if (threadIdx.x == 0) s_item = q_base + atomicAdd(ctr, 1u);
         __syncthreads();
         item = s_item;
     }
-    while (item < nitems) {
+    while (item < q_end) {
         prefilter_item<BF16, KSX, WAVES>(frag, frag_a8, items[item], item, Bl, stage_all, out);
         if (!item_ctr) break;
         __syncthreads();  // every wave is done with Bl and s_item
-        if (threadIdx.x == 0) s_item = atomicAdd(item_ctr, 1u);
+        if (threadIdx.x == 0) s_item = q_base + atomicAdd(ctr, 1u);
         __syncthreads();
         item = s_item;
     }
@@ -1718,27 +1732,53 @@ uint32_t seg_cols_for(uint32_t n) {
     return seg;
 }
 
-int build_items(mdet_ctx* ctx, uint32_t n_pad, uint32_t seg, uint4** items_dev, uint32_t* nitems) {
-    if (ctx->items_npad == n_pad && ctx->items_seg == seg && ctx->nitems) {
+// order 0: one list, row-block major.  order 1: PF_QUEUES lists (row blocks
+// rb with rb % PF_QUEUES == q), each ordered by column start then row block,
+// so the blocks of one XCD work on neighbouring column ranges together;
+// qbounds[0..PF_QUEUES] delimit the lists.
+int build_items(mdet_ctx* ctx, uint32_t n_pad, uint32_t seg, uint32_t order, uint4** items_dev, uint32_t* nitems,
+                uint32_t** qbounds_dev) {
+    if (ctx->items_npad == n_pad && ctx->items_seg == seg && ctx->items_order == order && ctx->nitems) {
         auto it = ctx->bufs.find("pairs.items");
-        if (it != ctx->bufs.end()) {
+        auto iq = ctx->bufs.find("pairs.item_qb");
+        if (it != ctx->bufs.end() && iq != ctx->bufs.end()) {
             *items_dev = static_cast<uint4*>(it->second.p);
+            *qbounds_dev = static_cast<uint32_t*>(iq->second.p);
             *nitems = ctx->nitems;
             return MDET_OK;
         }
     }
     std::vector<uint4> items;
+    std::vector<uint32_t> qb(PF_QUEUES + 1, 0);
     const uint32_t nrb = n_pad / M_ROWS;
-    for (uint32_t rb = 0; rb < nrb; ++rb)
+    auto row_items = [&](uint32_t rb, std::vector<uint4>& v) {
         for (uint32_t c = rb * M_ROWS; c < n_pad; c += seg)
-            items.push_back(make_uint4(rb * M_ROWS, rb * M_ROWS + M_ROWS, c, std::min<uint32_t>(c + seg, n_pad) | ITEM_TRI));
+            v.push_back(make_uint4(rb * M_ROWS, rb * M_ROWS + M_ROWS, c, std::min<uint32_t>(c + seg, n_pad) | ITEM_TRI));
+    };
+    if (order == 0) {
+        for (uint32_t rb = 0; rb < nrb; ++rb) row_items(rb, items);
+        for (int q = 1; q <= PF_QUEUES; ++q) qb[q] = (uint32_t)items.size();
+    } else {
+        for (int q = 0; q < PF_QUEUES; ++q) {
+            std::vector<uint4> qi;
+            for (uint32_t rb = q; rb < nrb; rb += PF_QUEUES) row_items(rb, qi);
+            std::stable_sort(qi.begin(), qi.end(), [](const uint4& x, const uint4& y) { return x.z < y.z; });
+            qb[q] = (uint32_t)items.size();
+            items.insert(items.end(), qi.begin(), qi.end());
+        }
+        qb[PF_QUEUES] = (uint32_t)items.size();
+    }
     *nitems = (uint32_t)items.size();
     MDET_TRY(ctx_buf(ctx, "pairs.items", items.size(), items_dev));
+    MDET_TRY(ctx_buf(ctx, "pairs.item_qb", PF_QUEUES + 1, qbounds_dev));
     MDET_HIP(hipMemcpyAsync(*items_dev, items.data(), items.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                            ctx->stream));
+    MDET_HIP(hipMemcpyAsync(*qbounds_dev, qb.data(), qb.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                             ctx->stream));
     MDET_HIP(hipStreamSynchronize(ctx->stream));
     ctx->items_npad = n_pad;
     ctx->items_seg = seg;
+    ctx->items_order = order;
     ctx->nitems = *nitems;
     return MDET_OK;
 }
@@ -1849,15 +1889,22 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         // a wave flushes once per item it has survivors in, plus once per
         // M_STAGE survivors; sized for ~128 survivors per contig at first
         const mdet_ctx::Tune tune = ctx->tune[precision];
-        // persistent prefilter: the resident blocks (8 waves per CU) pull items from a counter
+        // persistent prefilter: the resident blocks (8 waves per CU) pull items
+        // from a counter, or with per-XCD queues (qbounds) each from its
+        // queue's counter (the grid then covers every queue: a multiple of
+        // PF_QUEUES blocks)
         uint32_t* pf_ctr = nullptr;
+        uint32_t* qbounds = nullptr;
         auto pf_grid = [&](uint32_t nitems, int waves) {
-            return dim3(pf_ctr ? std::min<uint32_t>(nitems, (uint32_t)ctx->num_cus * (8 / waves)) : nitems);
+            const uint32_t resident = (uint32_t)ctx->num_cus * (8 / waves);
+            if (!pf_ctr) return dim3(nitems);
+            if (qbounds) return dim3((std::max<uint32_t>(resident, PF_QUEUES) + PF_QUEUES - 1) / PF_QUEUES * PF_QUEUES);
+            return dim3(std::min<uint32_t>(nitems, resident));
         };
         auto chunk_out = [&](uint32_t nitems, int attempt) -> int {
             if (tune.persistent) {
-                MDET_TRY(ctx_buf(ctx, "pairs.item_ctr", 1, &pf_ctr));
-                MDET_HIP(hipMemsetAsync(pf_ctr, 0, 4, ctx->stream));
+                MDET_TRY(ctx_buf(ctx, "pairs.item_ctr", PF_QUEUES * PF_CTR_STRIDE, &pf_ctr));
+                MDET_HIP(hipMemsetAsync(pf_ctr, 0, PF_QUEUES * PF_CTR_STRIDE * 4, ctx->stream));
             }
             if (attempt == 0)
                 chunk_cap = std::max<uint64_t>(
@@ -1944,19 +1991,21 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 uint4* items = ivf.items_dev;
                 uint32_t nitems = ivf.nitems;
                 if (!n_probe)
-                    MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), &items, &nitems));
+                    MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), 0, &items,
+                                         &nitems, &qbounds));
+                qbounds = nullptr;  // one queue
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (nitems && tnf_gram)
                     hipLaunchKernelGGL((k_prefilter_mfma<true, KS>), pf_grid(nitems, 4), dim3(256), 0, ctx->stream,
-                                       frag, frag_a, items, nitems, pf_ctr, co);
+                                       frag, frag_a, items, nitems, pf_ctr, (const uint32_t*)nullptr, co);
                 else if (nitems && n_probe)  // cluster items: at most ITEM_ROWS rows
                     hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E>), pf_grid(nitems, 4), dim3(256), 0,
-                                       ctx->stream, frag, frag_a, items, nitems, pf_ctr, co);
+                                       ctx->stream, frag, frag_a, items, nitems, pf_ctr, (const uint32_t*)nullptr, co);
                 else if (nitems)  // row-block items of M_ROWS rows
                     hipLaunchKernelGGL((k_prefilter_mfma<true, KS_E, M_GRAM_WAVES>), pf_grid(nitems, M_GRAM_WAVES),
                                        dim3(64 * M_GRAM_WAVES), 0, ctx->stream, frag, frag_a, items, nitems, pf_ctr,
-                                       co);
+                                       (const uint32_t*)nullptr, co);
             } else {
                 const bool bf = precision == MDET_BF16_REFINE;
                 const float R2 = gram_r2(bf);
@@ -1972,17 +2021,20 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                                        tnf_dev, perm, n, n_pad, R2, frag, frag_a8);
                 uint4* items;
                 uint32_t nitems;
-                MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), &items, &nitems));
+                const uint32_t order = tune.persistent ? tune.item_order : 0u;
+                MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), order, &items,
+                                     &nitems, &qbounds));
+                if (!order) qbounds = nullptr;
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
                 if (bf)
                     hipLaunchKernelGGL((k_prefilter_mfma<true, KS, M_GRAM_WAVES>), pf_grid(nitems, M_GRAM_WAVES),
                                        dim3(64 * M_GRAM_WAVES), 0, ctx->stream, frag, frag_a8, items, nitems, pf_ctr,
-                                       co);
+                                       (const uint32_t*)qbounds, co);
                 else
                     hipLaunchKernelGGL((k_prefilter_mfma<false, KS, M_GRAM_WAVES>), pf_grid(nitems, M_GRAM_WAVES),
                                        dim3(64 * M_GRAM_WAVES), 0, ctx->stream, frag, frag_a8, items, nitems, pf_ctr,
-                                       co);
+                                       (const uint32_t*)qbounds, co);
             }
             MDET_HIP(hipGetLastError());
             if (attempt == 0) MDET_TRY(tm.record(2));
@@ -2186,7 +2238,7 @@ int mdet_tune_get(mdet_ctx* ctx, int precision, mdet_tune* out) {
     if (precision < MDET_EXACT_F32 || precision > MDET_LEARNED || !out)
         MDET_FAIL(MDET_E_INVALID, "mdet_tune_get: bad precision %d or null output", precision);
     const mdet_ctx::Tune& t = ctx->tune[precision];
-    *out = mdet_tune{t.seg_cols, t.persistent, t.refine_blocks, 0u, t.ms_prefilter, t.ms_refine};
+    *out = mdet_tune{t.seg_cols, t.persistent, t.refine_blocks, t.item_order, t.ms_prefilter, t.ms_refine};
     return MDET_OK;
 }
 
@@ -2200,10 +2252,12 @@ int mdet_tune_set(mdet_ctx* ctx, int precision, const mdet_tune* in) {
     if (in->persistent > 1) MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: persistent must be 0 or 1");
     if (in->refine_blocks < 1 || in->refine_blocks > 64)
         MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: refine_blocks %u outside [1, 64]", in->refine_blocks);
+    if (in->item_order > 1) MDET_FAIL(MDET_E_RANGE, "mdet_tune_set: item_order must be 0 or 1");
     mdet_ctx::Tune& t = ctx->tune[precision];
     t.seg_cols = in->seg_cols;
     t.persistent = in->persistent;
     t.refine_blocks = in->refine_blocks;
+    t.item_order = in->item_order;
     t.ms_prefilter = in->ms_prefilter;
     t.ms_refine = in->ms_refine;
     return MDET_OK;
@@ -2250,20 +2304,24 @@ int autotune_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev
     };
     if (int rc = call(); rc != MDET_OK) return fail(rc);  // first-call allocations at the current shape
     // 1. prefilter shape
-    std::vector<std::pair<uint32_t, uint32_t>> shapes;  // (seg_cols, persistent)
+    struct Shape {
+        uint32_t seg, pers, order;
+    };
+    std::vector<Shape> shapes;
     if (precision == MDET_BF16_REFINE || precision == MDET_F16_REFINE) {
         for (uint32_t seg : {512u, 1024u, 2048u, 4096u})
-            for (uint32_t pers : {0u, 1u}) shapes.push_back({seg, pers});
+            for (const Shape sh : {Shape{seg, 0u, 0u}, Shape{seg, 1u, 0u}, Shape{seg, 1u, 1u}}) shapes.push_back(sh);
     } else if (precision == MDET_LEARNED) {
-        for (uint32_t pers : {0u, 1u}) shapes.push_back({t.seg_cols, pers});
+        for (uint32_t pers : {0u, 1u}) shapes.push_back({t.seg_cols, pers, t.item_order});
     }
     double best_pre = 0;
     if (!shapes.empty()) {
         best_pre = 1e300;
-        std::pair<uint32_t, uint32_t> best = shapes[0];
+        Shape best = shapes[0];
         for (const auto& sh : shapes) {
-            t.seg_cols = sh.first;
-            t.persistent = sh.second;
+            t.seg_cols = sh.seg;
+            t.persistent = sh.pers;
+            t.item_order = sh.order;
             double ms;
             if (int rc = median_of(&mdet_pair_stats::ms_prefilter, &ms); rc != MDET_OK) return fail(rc);
             if (ms < best_pre) {
@@ -2271,8 +2329,9 @@ int autotune_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev
                 best = sh;
             }
         }
-        t.seg_cols = best.first;
-        t.persistent = best.second;
+        t.seg_cols = best.seg;
+        t.persistent = best.pers;
+        t.item_order = best.order;
     }
     // 2. refine blocks per CU
     double best_ref = 1e300;
@@ -2289,7 +2348,7 @@ int autotune_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev
     t.refine_blocks = best_rb;
     t.ms_prefilter = best_pre;
     t.ms_refine = best_ref;
-    if (out) *out = mdet_tune{t.seg_cols, t.persistent, t.refine_blocks, 0u, t.ms_prefilter, t.ms_refine};
+    if (out) *out = mdet_tune{t.seg_cols, t.persistent, t.refine_blocks, t.item_order, t.ms_prefilter, t.ms_refine};
     return MDET_OK;
 }
 

@@ -1,6 +1,7 @@
 """Launch shapes of the pair stage: every prefilter column span, per-item or
-persistent prefilter blocks and refine grid returns exactly the same edge
-list, and the autotuner keeps one of its candidates."""
+persistent prefilter blocks (one item queue or one per XCD) and refine grid
+returns exactly the same edge list, and the autotuner keeps one of its
+candidates."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ from mdet_amd import workload
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(seg_cols=0, persistent=1, refine_blocks=64)
+DEFAULT = dict(seg_cols=0, persistent=1, refine_blocks=64, item_order=1)
 
 
 def _same(a, b):
@@ -25,8 +26,9 @@ def test_every_gram_launch_shape_gives_the_same_edges(ctx):
             base = ctx.tnf_edges(w.tnf, w.logsize, w.n, 0.95, 200, mode)
             print(f"mode {mode}: {len(base[0])} edges, {base[3].survivors} survivors", flush=True)
             assert len(base[0]) > 10000
-            shapes = [dict(seg_cols=s, persistent=p, refine_blocks=32) for s in (256, 512, 1024, 4096, 8192)
-                      for p in (0, 1)] + [dict(seg_cols=0, persistent=0, refine_blocks=r) for r in (1, 8, 64)]
+            shapes = [dict(seg_cols=s, persistent=p, refine_blocks=32, item_order=o) for s in (256, 512, 1024, 4096, 8192)
+                      for p, o in ((0, 0), (1, 0), (1, 1))] + \
+                     [dict(seg_cols=0, persistent=0, refine_blocks=r, item_order=0) for r in (1, 8, 64)]
             for sh in shapes:
                 ctx.tune_set(mode, **sh)
                 got = ctx.tnf_edges(w.tnf, w.logsize, w.n, 0.95, 200, mode)
@@ -47,7 +49,7 @@ def test_autotune_keeps_a_candidate_and_the_edges(ctx):
         base = ctx.tnf_edges(w.tnf, w.logsize, w.n, 0.95, 200, M.BF16_REFINE)
         t = ctx.autotune(w.tnf, w.logsize, w.n, 0.95, 200, M.BF16_REFINE, reps=1)
         print(f"autotuned: {t}", flush=True)
-        assert t["seg_cols"] in (512, 1024, 2048, 4096) and t["persistent"] in (0, 1)
+        assert t["seg_cols"] in (512, 1024, 2048, 4096) and t["persistent"] in (0, 1) and t["item_order"] in (0, 1)
         assert t["refine_blocks"] in (8, 16, 32, 64)
         assert t["ms_prefilter"] > 0 and t["ms_refine"] > 0
         assert ctx.tune_get(M.BF16_REFINE) == t
@@ -73,8 +75,8 @@ def test_learned_launch_shapes_give_the_same_edges(ctx):
         ctx.tune_set(M.LEARNED, **DEFAULT)
         kw = dict(mlp=mlp, radius2=r2, n_probe=3, n_clusters=9)
         base = ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, **kw)
-        for sh in (dict(seg_cols=1024, persistent=0, refine_blocks=32), dict(seg_cols=0, persistent=1,
-                                                                            refine_blocks=4)):
+        for sh in (dict(seg_cols=1024, persistent=0, refine_blocks=32, item_order=0),
+                   dict(seg_cols=0, persistent=1, refine_blocks=4, item_order=1)):
             ctx.tune_set(M.LEARNED, **sh)
             assert _same(ctx.tnf_edges(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, **kw), base), sh
         t = ctx.autotune(w.tnf, w.logsize, n, 0.95, 200, M.LEARNED, reps=1, **kw)
@@ -91,7 +93,7 @@ def test_learned_launch_shapes_give_the_same_edges(ctx):
 def test_tune_set_rejects_bad_shapes(ctx):
     ctx.tune_set(M.BF16_REFINE, **DEFAULT)
     for bad in (dict(seg_cols=100), dict(seg_cols=1000), dict(seg_cols=1 << 17), dict(persistent=2),
-                dict(refine_blocks=0), dict(refine_blocks=65)):
+                dict(refine_blocks=0), dict(refine_blocks=65), dict(item_order=2)):
         with pytest.raises(M.MdetError, match="mdet_tune_set"):
             ctx.tune_set(M.BF16_REFINE, **{**DEFAULT, **bad})
     with pytest.raises(M.MdetError, match="precision"):
