@@ -671,9 +671,15 @@ __device__ inline void stage_flush(const ChunkOut& o, uint32_t shard, uint32_t i
 
 // Chunks of shard s (offsets from k_shard_scan over the chunk counters) ->
 // (item, chunk slot) pairs for the sort; survivor total into *total.
+// Sort key of a chunk: its item index, or with items != nullptr the
+// 1024 x 1024 tile of the item's first row and column, (r_lo / 1024) << 16 |
+// c_lo / 1024, so that the refinement walks the survivors tile by tile and
+// the rows in flight on an XCD stay few (row-block items span up to 4096
+// columns; cluster items are compact already).
 __global__ __launch_bounds__(256) void k_chunk_list(const uint2* __restrict__ meta, unsigned long long cap,
                                                     const unsigned long long* __restrict__ offsets,
-                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                                    const uint4* __restrict__ items, uint32_t* __restrict__ key,
+                                                    uint32_t* __restrict__ slot,
                                                     unsigned long long* __restrict__ total) {
     __shared__ unsigned int bsum;
     if (threadIdx.x == 0) bsum = 0;
@@ -683,7 +689,12 @@ __global__ __launch_bounds__(256) void k_chunk_list(const uint2* __restrict__ me
     unsigned int sum = 0;
     for (unsigned long long k = threadIdx.x; k < c; k += blockDim.x) {
         const uint2 m = meta[(size_t)s * cap + k];
-        key[b + k] = m.x;
+        if (items) {
+            const uint4 it = items[m.x];
+            key[b + k] = ((it.x >> 10) << 16) | (it.z >> 10);
+        } else {
+            key[b + k] = m.x;
+        }
         slot[b + k] = (uint32_t)((size_t)s * cap + k);
         sum += m.y;
     }
@@ -1885,6 +1896,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         ChunkOut co{};
         uint32_t *ckey = nullptr, *cslot = nullptr;
         uint32_t items_launched = 0;
+        const uint4* tile_items = nullptr;  // row-block items: survivor chunks sorted by tile
         uint64_t nchunks = 0;
         // a wave flushes once per item it has survivors in, plus once per
         // M_STAGE survivors; sized for ~128 survivors per contig at first
@@ -2024,6 +2036,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                 const uint32_t order = tune.persistent ? tune.item_order : 0u;
                 MDET_TRY(build_items(ctx, n_pad, tune.seg_cols ? tune.seg_cols : seg_cols_for(n), order, &items,
                                      &nitems, &qbounds));
+                tile_items = n_pad <= (1u << 26) ? items : nullptr;  // the tile key fits 32 bits
                 if (!order) qbounds = nullptr;
                 MDET_TRY(chunk_out(nitems, attempt));
                 if (attempt == 0) MDET_TRY(tm.record(1));
@@ -2043,7 +2056,7 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                                soff, sstat);
             if (chunked)
                 hipLaunchKernelGGL(k_chunk_list, dim3(NSHARD), dim3(256), 0, ctx->stream, co.meta,
-                                   (unsigned long long)chunk_cap, soff, ckey, cslot, sstat + 2);
+                                   (unsigned long long)chunk_cap, soff, tile_items, ckey, cslot, sstat + 2);
             unsigned long long st2[3];
             {
                 void* st_host;
@@ -2074,8 +2087,10 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             MDET_TRY(ctx_buf(ctx, "pairs.cslot2", nchunks, &cslot2));
             MDET_TRY(ctx_buf(ctx, "pairs.ccnt", nchunks + 1, &ccnt));  // the scan reads one past the end
             MDET_TRY(ctx_buf(ctx, "pairs.coff", nchunks + 1, &coff));
+            // key bits: item indices, or the tile key (16 + bits of n_pad / 1024)
+            const uint64_t key_range = tile_items ? ((uint64_t)(n_pad >> 10) + 1) << 16 : (uint64_t)items_launched;
             int end_bit = 1;
-            while (end_bit < 32 && (1ull << end_bit) < items_launched) ++end_bit;
+            while (end_bit < 32 && (1ull << end_bit) < key_range) ++end_bit;
             size_t tmp = 0;
             MDET_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ckey, ckey2, cslot, cslot2, (int)nchunks, 0,
                                                         end_bit, ctx->stream));
