@@ -378,11 +378,18 @@ __global__ __launch_bounds__(256) void k_km_probe(const float* __restrict__ cent
         bd[t] = __builtin_inff();
         bx[t] = 0xffffffffu;
     }
-    for (uint32_t b = lane; b < k; b += 64) {
-        if (counts[b] == 0) continue;
-        float dist = -1.f;
-        if (b != a) {
-            const float4* cb = reinterpret_cast<const float4*>(cent + (size_t)b * E_D);
+    // KM_SCAN candidates b = b0 + 64 u per lane and round, loaded together
+    // (clamped index, masked after) so their latencies overlap; inserted in
+    // increasing b as before
+    constexpr uint32_t KM_SCAN = 4;
+    for (uint32_t b0 = lane; b0 < k; b0 += 64 * KM_SCAN) {
+        float dists[KM_SCAN];
+#pragma unroll
+        for (uint32_t u = 0; u < KM_SCAN; ++u) {
+            const uint32_t b = b0 + 64 * u;
+            const uint32_t bc = b < k ? b : k - 1;
+            const bool nonempty = counts[bc] != 0;
+            const float4* cb = reinterpret_cast<const float4*>(cent + (size_t)bc * E_D);
             float s = 0.f;
 #pragma unroll
             for (int v = 0; v < E_D / 4; ++v) {
@@ -394,11 +401,13 @@ __global__ __launch_bounds__(256) void k_km_probe(const float* __restrict__ cent
                 s = fmaf(t2, t2, s);
                 s = fmaf(t3, t3, s);
             }
-            dist = s;
+            dists[u] = (b < k && nonempty) ? (b == a ? -1.f : s) : __builtin_inff();
         }
-        if (dist < bd[KM_MAX_PROBE - 1]) {
-            float cd = dist;
-            uint32_t cx = b;
+#pragma unroll
+        for (uint32_t u = 0; u < KM_SCAN; ++u) {
+            if (!(dists[u] < bd[KM_MAX_PROBE - 1])) continue;
+            float cd = dists[u];
+            uint32_t cx = b0 + 64 * u;
 #pragma unroll
             for (uint32_t t = 0; t < KM_MAX_PROBE; ++t) {
                 if (cd < bd[t]) {
