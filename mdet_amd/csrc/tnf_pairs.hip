@@ -11,7 +11,13 @@
 //            distance threshold (+ the rounding error of the mode):
 //              EXACT_F32   FP32 sum of squared differences, VALU tiles
 //              *_REFINE    Gram d^2 = |a|^2 + |b|^2 - 2 a.b on MFMA
-//                          (v_mfma_f32_32x32x16_{bf16,f16}), FP32 accumulate
+//                          (v_mfma_f32_32x32x16_{bf16,f16}), FP32 accumulate,
+//                          over (256-row block, column segment) items taken
+//                          by persistent blocks from per-XCD queues; the
+//                          survivors leave in tagged chunks, sorted into a
+//                          tile-by-tile list
+//              LEARNED     the same kernel over the pairs of cluster pairs
+//                          chosen by the learned embeddings (tnf_ivf.hip)
 //   refine   per survivor: FP32 sum of squared differences, FP32 model via
 //            the Chebyshev form; a pair whose FP32 preProb lies within EPS of
 //            the threshold (or whose model-1 value lies within EPS of ln 9,
@@ -1728,12 +1734,15 @@ int sort_by_composition(mdet_ctx* ctx, const float* tnf_dev, uint32_t n, uint32_
 }
 
 // Prefilter items for the MFMA kernel: (row block, column begin, column end),
-// row-block major, each row block's columns from its diagonal in segments of
-// seg columns.  The survivor-heavy diagonal item of every row block is then
-// one of ~n/seg items in flight, so extraction work interleaves with
-// MFMA-only blocks on every CU.  (A segment-major order with an XCD-aware block map cut
-// the prefilter's L2-miss traffic from 19 GB to 2.9 GB at 200k contigs but ran
-// 8% slower: the kernel is bound by instruction issue, not by those fetches.)
+// each row block's columns from its diagonal in segments of seg columns.
+// With one queue (row-block major) the survivor-heavy diagonal item of every
+// row block is one of ~n/seg items in flight; with per-XCD queues (column
+// start major within a queue of every 8th row block) the diagonal items still
+// come spread over time and XCDs, while an XCD's blocks share column ranges
+// in its L2: memory-side traffic 19.4 -> 3.5 GB and 10% faster at 200k
+// contigs.  (An earlier column-major order with one block per item and an
+// XCD-aware block map ran the diagonal items back to back on one XCD: 8%
+// slower.)
 // Default column span by size: the power of two nearest n / 100 in [512,
 // 4096] (the autotuner's picks measured at 50k, 200k and 1M contigs were
 // 512, 2048 and 4096, with persistent blocks).
@@ -2284,9 +2293,10 @@ namespace {
 
 // Times the launch shapes of the pair stage on this input and keeps the
 // fastest in ctx->tune[precision]: first the prefilter shape (item column
-// span for the Gram modes x one block per item or persistent blocks), by
-// the median prefilter stage time over reps calls, then the refine blocks
-// per CU at that shape, by the median refine stage time.  Every shape gives
+// span for the Gram modes x {one block per item, persistent blocks with one
+// queue, persistent blocks with per-XCD queues}), by the median prefilter
+// stage time over reps calls, then the refine blocks per CU at that shape,
+// by the median refine stage time.  Every shape gives
 // the same edge list; the last call's list stays in ctx.
 int autotune_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, uint32_t n, double cutoff,
                   uint32_t max_edges, int precision, mdet_mlp* mlp, float radius2, uint32_t n_clusters,
