@@ -5,12 +5,13 @@
 // embeddings do not form tiles compact enough to skip 32x32 blocks of pairs
 // by bounds.  They do cluster: k-means with one centroid per ~192 contigs puts
 // most partner pairs in the same cluster, and almost all of the rest in one of
-// the few clusters whose centroids are nearest.  So the learned path evaluates
-// its embedding Gram only on the pairs of probed cluster pairs:
+// the few clusters whose centroids are nearest.  So the learned path runs its
+// prefilter (the BF16 TNF Gram, or the embedding Gram with a radius) only on
+// the pairs of probed cluster pairs:
 //   1. k-means (Lloyd) on a sample of the rows, 4-8 iterations, distances on
-//      FP16 MFMA; centroid sums in 2^-20 fixed point (integer atomics: the
-//      result does not depend on the order of the additions), then one
-//      assignment of every row;
+//      FP16 MFMA (hi/lo split); centroid sums in 2^-20 fixed point (integer
+//      atomics: the result does not depend on the order of the additions),
+//      then one assignment of every row;
 //   2. rows sorted by cluster (radix sort, stable), cluster offsets;
 //   3. per cluster the n_probe nearest non-empty centroids (itself first);
 //   4. on the host: the distinct unordered cluster pairs (a, b) with b in a's
@@ -199,8 +200,10 @@ __global__ __launch_bounds__(256) void k_km_cfrag(const float* __restrict__ cent
 
 // Nearest centroid of the points q * stride, q < m.  Block = 32 R points x W
 // waves: every wave holds the points' A fragments and scans the centroid
-// tiles t = w, w + W, ...; per element one AND-OR (key) and a running
-// minimum; the 32 lanes of a column half then reduce each row, and the W
+// tiles t = w, w + W, ... (7 MFMAs per 32 x 32 tile: hi.hi, hi.lo, lo.hi
+// over two k-steps each, and the norm slots); per element one AND-OR (key)
+// and a running minimum; the 32 lanes of a column half then reduce each
+// row, and the W
 // waves' results are merged in LDS.  mode 1: counts[best] += 1; mode 2: also
 // the point's coordinates into the fixed-point sums of its centroid.  err
 // bit 0: a point outside the frame (only non-finite or overflowing rows).
