@@ -143,6 +143,13 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
                 // the whole word belongs to contig cs + s: 16 windows, N mask only
                 uint32_t valid = ~bad & 0xFFFFu;
                 uint32_t* h = hist[s];
+                if (__builtin_expect(valid == 0xFFFFu, 1)) {
+                    // N-free word: constant shifts, so each window is a bit-field
+                    // extract, an address add and the LDS add
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) atomicAdd(&h[(uint32_t)(x >> (26 + 2 * k)) & 0xFFu], 1u);
+                    valid = 0;
+                }
                 while (valid) {
                     const int k = __builtin_ctz(valid);
                     valid &= valid - 1;
