@@ -49,7 +49,7 @@ CUTOFF, MAX_EDGES = 0.95, 200
 # the same counters on a plain 1 GiB copy (x2.0 fetch, x1.0 write); summary and
 # commands in profiles/pairs_200k_pmc_traffic.md.  Counters cannot be read in
 # the timed run, so the figure is carried here for the workload it was taken on.
-PMC_TRAFFIC = {"c3_200k": {"fetch_kib": 1643680.6, "write_kib": 168738.8, "fetch_scale": 2.0, "write_scale": 1.0}}
+PMC_TRAFFIC = {"c3_200k": {"fetch_kib": 1644350.8, "write_kib": 168758.4, "fetch_scale": 2.0, "write_scale": 1.0}}
 
 
 def log(msg):
@@ -137,7 +137,8 @@ def cpu_count_baseline(w, cfg_name, budget_contigs=30000):
                       f"openmp/TNF.cpp counts), its own timer around the TNF loop: {secs:.3f} s"}
 
 
-def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, autotune=True, use_radius=False):
+def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, autotune=True, use_radius=False,
+                knn_study=False):
     """Train the filter on pairs the exact path labels, then time the learned
     edge list (cluster index over the embeddings, n_probe nearest clusters;
     within the probed cluster pairs the BF16 TNF Gram decides, or with
@@ -172,6 +173,10 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, au
     got = f.astype(np.uint64) * n + t.astype(np.uint64)
     hit = np.isin(exact, got)
     mlp.close()
+    # the per-row top-k form of the filter (k nearest embedding neighbours per column tile), measured
+    # by brute force for comparison with the cluster index
+    knn_points = [(128, 0), (256, 0), (512, 0), (256, n // 4), (64, n // 16)] if knn_study else []
+    knn = {"trained": [L.knn_candidate_recall(emb, exact, k, t_) for k, t_ in knn_points]}
     # the same filter with an untrained network (what the training buys)
     raw = M.Mlp(ctx, seed=7)
     raw.set_input_norm(mean, istd)
@@ -188,6 +193,9 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, au
     raw_sec = (time.perf_counter() - t0) / 3
     fr, tr, _, str_ = ctx.tnf_edges(w.tnf, w.logsize, n, CUTOFF, MAX_EDGES, M.LEARNED, **kw_raw)
     raw_hit = np.isin(exact, fr.astype(np.uint64) * n + tr.astype(np.uint64))
+    if knn_points:
+        raw_emb = L.embeddings(ctx, raw, w.tnf, n)
+        knn["untrained"] = [L.knn_candidate_recall(raw_emb, exact, k, t_) for k, t_ in knn_points]
     raw.close()
     untrained = {"ms_per_step": raw_sec * 1e3, "candidate_pairs": str_.pairs_evaluated,
                  "survivors_exact_evaluated": str_.survivors,
@@ -201,6 +209,7 @@ def run_learned(ctx, w, fb, tb, train_steps, steps, n_probe, quantile=0.9999, au
             "edges_not_in_exhaustive_set": int((~np.isin(got, exact)).sum()),
             "within_cluster_pairs": "embedding radius" if use_radius else "BF16 TNF Gram prefilter",
             "untrained_network_same_probes": untrained,
+            **({"per_row_topk_form": knn} if knn_points else {}),
             "radius2": r2, "radius_quantile": quantile if use_radius else None, "tune": tune,
             "training": {"pairs": int(len(pi)), "positives": int(y.sum()), "steps": train_steps, "batch": 512,
                          "seconds": train_s, "ms_per_step": 1e3 * train_s / max(1, train_steps),
@@ -268,6 +277,9 @@ def main():
     ap.add_argument("--learned-radius", action="store_true",
                     help="learned filter: embedding radius test within the probed cluster pairs "
                          "(default: the BF16 TNF Gram decides there)")
+    ap.add_argument("--knn-study", action="store_true",
+                    help="also measure the per-row top-k form of the learned filter (k nearest embedding "
+                         "neighbours per column tile, brute force) for the trained and untrained network")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the BF16 and learned paths at the default launch shapes")
     ap.add_argument("--n-probe", type=int, default=0,
@@ -282,6 +294,10 @@ def main():
         copy_probe(args.copy_probe_mib)
         return
 
+    if args.knn_study:
+        # torch's HIP runtime has to claim the device before the library's context does
+        import torch
+        torch.cuda.init()
     ctx = M.Ctx(0)
     log(f"generating {args.workload}")
     w = W.make(ctx, args.workload, n=args.n, seed=1, keep_stream=True)
@@ -334,7 +350,7 @@ def main():
         log(f"learned filter: {args.train_steps} training steps")
         n_probe = args.n_probe or (20 if w.n <= 262144 else 32)
         lm = run_learned(ctx, w, fb, tb, args.train_steps, args.steps, n_probe, autotune=not args.no_autotune,
-                         use_radius=args.learned_radius)
+                         use_radius=args.learned_radius, knn_study=args.knn_study)
         lm["speedup_vs_bf16_refine"] = lm["pairs_per_s"] / modes["bf16_refine"]["pairs_per_s"]
         modes["learned"] = lm
 

@@ -102,6 +102,42 @@ def embeddings(ctx: Ctx, mlp: Mlp, tnf: DevArray, n: int) -> np.ndarray:
     return out
 
 
+def knn_candidate_recall(emb: np.ndarray, exact_keys: np.ndarray, k: int, tile: int = 0,
+                         rows: int = 2048, device: str = "cuda") -> dict:
+    """Candidate pairs and edge recall of the per-row top-k form of the filter, for comparison with
+    the cluster index: every contig keeps its k nearest embedding neighbours within each column
+    tile of `tile` contigs (0: all columns), and a pair is a candidate if either endpoint kept the
+    other.  Brute force through torch (on the GPU, torch has to be initialised before the library's
+    context in the same process): a measurement of that alternative, not a product path.
+    exact_keys = min * n + max of the exhaustive edge list."""
+    import torch
+
+    dev = torch.device(device)
+    n = emb.shape[0]
+    e = torch.from_numpy(np.ascontiguousarray(emb, dtype=np.float32)).to(dev)
+    nrm = (e * e).sum(1)
+    tile = tile or n
+    keys = []
+    for r0 in range(0, n, rows):
+        r1 = min(n, r0 + rows)
+        rid = torch.arange(r0, r1, device=dev)
+        picks = []
+        for c0 in range(0, n, tile):
+            c1 = min(n, c0 + tile)
+            d2 = nrm[r0:r1, None] + nrm[None, c0:c1] - 2.0 * (e[r0:r1] @ e[c0:c1].T)
+            if c0 < r1 and r0 < c1:  # a contig is not its own neighbour
+                d2 = d2.masked_fill(rid[:, None] == torch.arange(c0, c1, device=dev)[None, :], float("inf"))
+            picks.append(torch.topk(d2, min(k, c1 - c0 - 1), dim=1, largest=False).indices + c0)
+        idx = torch.cat(picks, 1)
+        row = rid[:, None].expand_as(idx)
+        keys.append((torch.minimum(row, idx) * n + torch.maximum(row, idx)).reshape(-1))
+    cand = torch.unique(torch.cat(keys))
+    del keys
+    ex = torch.from_numpy(exact_keys.astype(np.int64)).to(dev)
+    recall = float(torch.isin(ex, cand).float().mean()) if ex.numel() else 1.0
+    return {"k": k, "tile": tile, "candidate_pairs": int(cand.numel()), "edge_recall": recall}
+
+
 def calibrate_radius2(emb: np.ndarray, pi: np.ndarray, pj: np.ndarray, quantile: float = 0.9999,
                       slack: float = 1.05) -> float:
     """Squared embedding radius holding the given fraction of the positive pairs."""

@@ -1,5 +1,6 @@
 """CPU tests of the product's host logic: C ABI exports, FASTA reader, 2-bit
-packer, synthetic generator.  No device calls."""
+packer, synthetic generator, the learned filter's top-k comparison helper.
+No device calls."""
 import os
 import re
 
@@ -7,6 +8,7 @@ import numpy as np
 import pytest
 
 import mdet_amd as M
+from mdet_amd import learned as L
 from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -114,3 +116,29 @@ def test_generator_genomes_have_distinct_composition():
 def test_generator_rejects_bad_config():
     with pytest.raises(M.MdetError):
         M.gen_lengths(M.gen_config(n_genomes=0), 3)
+
+
+def test_per_row_topk_measurement_against_brute_force():
+    """The top-k comparison helper: k = n - 1 makes every pair a candidate; k = 3 over column tiles of
+    50 contigs matches a numpy brute force of the same rule (distinct distances, so no ties)."""
+    rng = np.random.default_rng(5)
+    n = 300
+    emb = rng.standard_normal((n, 32)).astype(np.float32)
+    iu = np.triu_indices(n, 1)
+    keys = (iu[0] * n + iu[1]).astype(np.int64)
+    full = L.knn_candidate_recall(emb, keys, n - 1, device="cpu")
+    assert full["candidate_pairs"] == n * (n - 1) // 2 and full["edge_recall"] == 1.0
+    d2 = ((emb[:, None, :].astype(np.float64) - emb[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d2, np.inf)
+    want = set()
+    for c0 in range(0, n, 50):
+        srt = np.sort(d2[:, c0:c0 + 50], axis=1)
+        assert (srt[:, 3] - srt[:, 2]).min() > 1e-3  # the 3rd/4th ranks are far apart at FP32 resolution
+        nb = np.argsort(d2[:, c0:c0 + 50], axis=1, kind="stable")[:, :3] + c0
+        for i in range(n):
+            for j in nb[i]:
+                want.add(min(i, j) * n + max(i, j))
+    sample = keys[::7]
+    got = L.knn_candidate_recall(emb, sample, 3, tile=50, device="cpu")
+    assert got["candidate_pairs"] == len(want)
+    assert got["edge_recall"] == pytest.approx(np.isin(sample, np.fromiter(want, np.int64)).mean())
