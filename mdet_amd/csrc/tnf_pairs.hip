@@ -2143,18 +2143,13 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             MDET_HIP(hipGetLastError());
         }
         MDET_TRY(tm.record(3));
-        unsigned long long rch[2];
-        {
-            void* st_host;
-            MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
-            MDET_HIP(hipMemcpyAsync(st_host, rc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
-            MDET_HIP(hipStreamSynchronize(ctx->stream));
-            memcpy(rch, st_host, sizeof rch);
-        }
+        // no host round trip here: the select buffers are sized by the survivor
+        // count (a bound on the kept candidates), so the select stage queues
+        // behind k_refine; the kept and adjudicated counts are read with the
+        // select stage's flags
         const uint64_t nc = ns;  // candidate records (kept ones flagged)
         ctx->last_pairs = ns;
-        st.candidates = rch[0];
-        st.adjudicated = rch[1];
+        st.candidates = st.adjudicated = 0;
 
         // -- select
         uint32_t *bcur, *out_cnt, *iperm = nullptr;
@@ -2168,8 +2163,8 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
         MDET_TRY(ctx_buf(ctx, "sel.outcnt", (size_t)n + 1, &out_cnt));
         MDET_TRY(ctx_buf(ctx, "sel.rbeg", (size_t)n, &rbeg));
         MDET_TRY(ctx_buf(ctx, "sel.eoff", (size_t)n + 1, &eoff));
-        MDET_TRY(ctx_buf(ctx, "sel.ent", std::max<uint64_t>(2 * st.candidates, 1), &ent));
-        MDET_TRY(ctx_buf(ctx, "sel.adj", std::max<uint64_t>(2 * st.candidates, 1), &adj));
+        MDET_TRY(ctx_buf(ctx, "sel.ent", std::max<uint64_t>(2 * nc, 1), &ent));
+        MDET_TRY(ctx_buf(ctx, "sel.adj", std::max<uint64_t>(2 * nc, 1), &adj));
         MDET_TRY(ctx_buf(ctx, "sel.err", 4, &err));
         capped = reinterpret_cast<unsigned long long*>(err + 2);
         const uint32_t big_cap = std::min<uint32_t>(n, 1u << 20);
@@ -2196,14 +2191,24 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
                                max_edges, tnf_dev, logsize_dev, out_cnt, rbeg, err, capped);
             MDET_HIP(hipGetLastError());
             MDET_TRY(exclusive_scan_u32_to_u64(ctx, out_cnt, eoff, n));
-            unsigned long long errh[2];
+            // one round trip: select flags and capped rows, kept and adjudicated
+            // candidates (k_refine's counters), edge count
+            unsigned long long errh[2], rch[2];
+            uint64_t ne;
             {
                 void* st_host;
                 MDET_TRY(ctx_host_stage(ctx, 64, &st_host));
-                MDET_HIP(hipMemcpyAsync(st_host, err, 16, hipMemcpyDeviceToHost, ctx->stream));
+                char* h = static_cast<char*>(st_host);
+                MDET_HIP(hipMemcpyAsync(h, err, 16, hipMemcpyDeviceToHost, ctx->stream));
+                MDET_HIP(hipMemcpyAsync(h + 16, rc, 16, hipMemcpyDeviceToHost, ctx->stream));
+                MDET_HIP(hipMemcpyAsync(h + 32, eoff + n, 8, hipMemcpyDeviceToHost, ctx->stream));
                 MDET_HIP(hipStreamSynchronize(ctx->stream));
-                memcpy(errh, st_host, 16);
+                memcpy(errh, h, 16);
+                memcpy(rch, h + 16, 16);
+                memcpy(&ne, h + 32, 8);
             }
+            st.candidates = rch[0];
+            st.adjudicated = rch[1];
             if ((unsigned int)errh[0] & 1u)
                 MDET_FAIL(MDET_E_RANGE, "mdet_tnf_edges: a contig has more than %d candidate partners", SEL_CAP);
             if ((unsigned int)errh[0] & 2u)
@@ -2211,8 +2216,6 @@ int edges_impl(mdet_ctx* ctx, const float* tnf_dev, const double* logsize_dev, u
             if ((unsigned int)errh[0] & 4u)
                 MDET_FAIL(MDET_E_CAPACITY, "mdet_tnf_edges: more than %u rows need the block-wide selection", big_cap);
             st.rows_capped = errh[1];
-            uint64_t ne;
-            MDET_TRY(read_scalar(ctx, eoff + n, &ne));
             uint32_t *from, *to;
             float* s;
             MDET_TRY(ctx_buf(ctx, "out.from", std::max<uint64_t>(ne, 1), &from));
