@@ -12,8 +12,8 @@
 //     previous word, which the neighbouring lane loads anyway (L1/L2 hit).
 //   * Thread t reads words t, t+256, t+512, t+768 of the chunk: each load
 //     instruction of a wave covers 256 contiguous bytes.
-//   * Windows are binned by their raw 8-bit code into a block-shared LDS
-//     histogram per contig segment of the chunk; the fold to the 136 canonical
+//   * Windows are binned by their raw 8-bit code into block-shared LDS
+//     histograms per contig segment of the chunk; the fold to the 136 canonical
 //     columns (code and reverse complement) happens once per segment at flush,
 //     so the per-base work is extract + mask test + one LDS add.
 //   * N bases come as a sorted run list; the block looks up the runs that
@@ -28,7 +28,14 @@ namespace {
 constexpr int CNT_THREADS = 256;
 constexpr int CNT_WORDS = 4;  // words per thread per chunk
 constexpr uint64_t CNT_CHUNK = (uint64_t)CNT_THREADS * CNT_WORDS * 16;
-constexpr int CNT_MAXSEG = 16;  // contig segments histogrammed per pass
+constexpr int CNT_MAXSEG = 4;  // contig segments histogrammed per pass
+// Each segment's histogram exists in CNT_COPIES copies, lane l adding into copy
+// l % CNT_COPIES: same-bin adds of one wave instruction (frequent: 64 windows
+// over 256 codes) spread over the copies, and the copy stride of 264 words
+// (8 banks apart) spreads them over the LDS banks.  4 segments x 4 copies keep
+// the block at 19 KB of LDS; more segments or copies per pass cost more in
+// zeroing, folding and occupancy than they save.
+constexpr int CNT_COPIES = 4, CNT_CSTRIDE = 264;
 
 __constant__ int16_t c_colmap[256] = {
 #define M(i) mdet::kColumnMap.col[i]
@@ -98,7 +105,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
                                                               const uint64_t* __restrict__ base_off,
                                                               const longlong4* __restrict__ meta,
                                                               uint32_t* __restrict__ counts) {
-    __shared__ uint32_t hist[CNT_MAXSEG][256];
+    __shared__ uint32_t hist[CNT_MAXSEG][CNT_COPIES * CNT_CSTRIDE];
     __shared__ uint32_t fold[CNT_MAXSEG][136];
     __shared__ uint64_t seg_lo[CNT_MAXSEG + 1];
 
@@ -113,7 +120,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
     for (int64_t cs = c_first; cs <= c_last; cs += CNT_MAXSEG) {
         const int nseg = (int)min<int64_t>(CNT_MAXSEG, c_last - cs + 1);
         if (tid <= nseg) seg_lo[tid] = base_off[cs + tid];
-        for (int i = tid; i < CNT_MAXSEG * 256; i += CNT_THREADS) (&hist[0][0])[i] = 0;
+        for (int i = tid; i < nseg * CNT_COPIES * CNT_CSTRIDE; i += CNT_THREADS) (&hist[0][0])[i] = 0;
         for (int i = tid; i < CNT_MAXSEG * 136; i += CNT_THREADS) (&fold[0][0])[i] = 0;
         __syncthreads();
         const uint64_t pass_lo = seg_lo[0], pass_hi = seg_lo[nseg];
@@ -142,7 +149,7 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
             if (s < nseg && p0 >= seg_lo[s] + 3 && p0 + 16 <= seg_end && p0 >= chunk_lo && p0 + 16 <= chunk_hi) {
                 // the whole word belongs to contig cs + s: 16 windows, N mask only
                 uint32_t valid = ~bad & 0xFFFFu;
-                uint32_t* h = hist[s];
+                uint32_t* h = hist[s] + (tid % CNT_COPIES) * CNT_CSTRIDE;
                 if (__builtin_expect(valid == 0xFFFFu, 1)) {
                     // N-free word: constant shifts, so each window is a bit-field
                     // extract, an address add and the LDS add
@@ -162,7 +169,9 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
                     while (s < nseg && seg_lo[s + 1] <= p) ++s;
                     const bool ok = (p >= chunk_lo) && (p < chunk_hi) && (p >= pass_lo) && (s < nseg) &&
                                     (p >= seg_lo[s] + 3) && (((bad >> k) & 1u) == 0);
-                    if (ok) atomicAdd(&hist[s][(uint32_t)(x >> (26 + 2 * k)) & 0xFFu], 1u);
+                    if (ok)
+                        atomicAdd(&hist[s][(tid % CNT_COPIES) * CNT_CSTRIDE + ((uint32_t)(x >> (26 + 2 * k)) & 0xFFu)],
+                                  1u);
                 }
             }
         }
@@ -170,7 +179,9 @@ __global__ __launch_bounds__(CNT_THREADS) void k_count_packed(const uint32_t* __
         // fold raw codes into canonical columns, then one global add per
         // nonzero column (a contig may span several chunks)
         for (int s = 0; s < nseg; ++s) {
-            uint32_t v = hist[s][tid];
+            uint32_t v = 0;
+#pragma unroll
+            for (int c = 0; c < CNT_COPIES; ++c) v += hist[s][c * CNT_CSTRIDE + tid];
             if (v) atomicAdd(&fold[s][c_colmap[tid]], v);
         }
         __syncthreads();
