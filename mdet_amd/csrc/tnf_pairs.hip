@@ -277,32 +277,49 @@ __device__ inline void shard_put(const SurvOut& o, uint32_t shard, unsigned long
 }
 
 // offsets[s] = sum of min(cnt, cap) of shards < s; offsets[NSHARD] = total;
-// stat[0] = total, stat[1] = largest raw count (overflow check).
+// stat[0] = total, stat[1] = largest raw count (overflow check).  One block:
+// thread t owns shards 2t and 2t + 1; wave scans, then a scan of the 16 wave
+// totals (integer sums, so the result is the serial prefix sum).
+static_assert(NSHARD == 2 * 1024, "k_shard_scan gives each of its 1024 threads two shards");
 __global__ __launch_bounds__(1024) void k_shard_scan(const unsigned long long* __restrict__ cnt,
                                                      unsigned long long cap, unsigned long long* __restrict__ offsets,
                                                      unsigned long long* __restrict__ stat) {
-    __shared__ unsigned long long sv[NSHARD];
-    __shared__ unsigned long long smax[1024];
-    const int t = threadIdx.x;
-    unsigned long long mx = 0;
-    for (int s = t; s < NSHARD; s += 1024) {
-        unsigned long long c = cnt[(size_t)s * SHARD_STRIDE];
-        mx = c > mx ? c : mx;
-        sv[s] = c < cap ? c : cap;
+    __shared__ unsigned long long wsum[16], wmax[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const unsigned long long c0 = cnt[(size_t)(2 * t) * SHARD_STRIDE], c1 = cnt[(size_t)(2 * t + 1) * SHARD_STRIDE];
+    const unsigned long long v0 = c0 < cap ? c0 : cap, v1 = c1 < cap ? c1 : cap;
+    unsigned long long x = v0 + v1, m = c0 > c1 ? c0 : c1;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+        const unsigned long long z = __shfl_xor(m, off, 64);
+        m = z > m ? z : m;
     }
-    smax[t] = mx;
+    if (lane == 63) wsum[w] = x;
+    if (lane == 0) wmax[w] = m;
     __syncthreads();
-    if (t == 0) {
-        unsigned long long acc = 0, m = 0;
-        for (int s = 0; s < NSHARD; ++s) {
-            offsets[s] = acc;
-            acc += sv[s];
+    if (w == 0) {
+        const unsigned long long own = lane < 16 ? wsum[lane] : 0ull;
+        unsigned long long sc = own, mm = lane < 16 ? wmax[lane] : 0ull;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const unsigned long long y = __shfl_up(sc, off, 64);
+            if (lane >= off) sc += y;
+            const unsigned long long z = __shfl_xor(mm, off, 64);
+            mm = z > mm ? z : mm;
         }
-        offsets[NSHARD] = acc;
-        for (int k = 0; k < 1024; ++k) m = smax[k] > m ? smax[k] : m;
-        stat[0] = acc;
-        stat[1] = m;
+        if (lane < 16) wsum[lane] = sc - own;  // exclusive prefix of wave `lane`
+        if (lane == 15) {
+            offsets[NSHARD] = sc;
+            stat[0] = sc;
+        }
+        if (lane == 0) stat[1] = mm;
     }
+    __syncthreads();
+    const unsigned long long base = wsum[w] + (x - (v0 + v1));
+    offsets[2 * t] = base;
+    offsets[2 * t + 1] = base + v0;
 }
 
 __global__ __launch_bounds__(256) void k_compact(const uint2* __restrict__ sbuf, unsigned long long cap,
